@@ -1,0 +1,286 @@
+"""ctypes bindings for the CPU oracle (liboracle.so) and the reference harness
+(_ref/libmptc_ref.so).  TEST INFRASTRUCTURE ONLY: imported by tests/, the
+smoke check and the bench's CPU-baseline leg, always as the checker.
+
+liboracle.so is the C restatement in mptc_oracle.c; libmptc_ref.so is the
+reference encoder compiled from its own sources by build_ref.sh (present
+wherever it was built, absent otherwise).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libmptc_ref.so")
+
+_u8p = C.POINTER(C.c_uint8)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_i32p = C.POINTER(C.c_int)
+_f64p = C.POINTER(C.c_double)
+
+
+class SynthCfg(C.Structure):
+    """mptc_synth_cfg of mptc_amd/csrc/framegen.h"""
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("seed", C.c_uint64),
+                ("n_rects", C.c_int32), ("vel_mode", C.c_int32), ("vel", C.c_int32),
+                ("noise_amp", C.c_int32), ("cut_permille", C.c_int32)]
+
+
+def build_oracle(verbose: bool = False) -> str:
+    """Compiles liboracle.so with gcc (no fused multiply-add, no fast-math)."""
+    src = os.path.join(HERE, "mptc_oracle.c")
+    tmp = f"{ORACLE_SO}.tmp{os.getpid()}"
+    cmd = ["gcc", "-std=c11", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+           "-Wall", "-o", tmp, src, "-lm"]
+    subprocess.run(cmd, check=True, capture_output=not verbose)
+    os.replace(tmp, ORACLE_SO)
+    return ORACLE_SO
+
+
+def build_ref(verbose: bool = False) -> str | None:
+    """Builds _ref/libmptc_ref.so from the reference sources when they are present."""
+    subprocess.run(["sh", os.path.join(HERE, "build_ref.sh")], check=True,
+                   capture_output=not verbose)
+    return REF_SO if os.path.exists(REF_SO) else None
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            build_oracle()
+        L = C.CDLL(path)
+        L.orc_init.restype = None
+        L.orc_fit_block.argtypes = [_u8p]
+        L.orc_fit_block.restype = C.c_uint64
+        L.orc_fit_frame.argtypes = [_u8p, C.c_int, C.c_int, _u64p]
+        L.orc_fit_frame.restype = None
+        L.orc_to_five_bits.argtypes = [C.c_uint8]
+        L.orc_to_five_bits.restype = C.c_uint8
+        L.orc_to_six_bits.argtypes = [C.c_uint8]
+        L.orc_to_six_bits.restype = C.c_uint8
+        L.orc_p2l.argtypes = [C.c_uint64, _u8p]
+        L.orc_p2l.restype = None
+        L.orc_l2p.argtypes = [_u8p]
+        L.orc_l2p.restype = C.c_uint64
+        L.orc_error.argtypes = [_u8p, _u8p]
+        L.orc_error.restype = C.c_int
+        L.orc_refit.argtypes = [_u8p, _u8p]
+        L.orc_refit.restype = None
+        L.orc_eval_candidate.argtypes = [C.c_uint64, _u8p, C.c_uint32, _i32p, _i32p, _u8p]
+        L.orc_eval_candidate.restype = C.c_int
+        L.orc_reencode.argtypes = [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _u64p,
+                                   _u64p, _u8p, _u32p, _u8p, _u8p]
+        L.orc_reencode.restype = C.c_int
+        L.orc_psnr_logical.argtypes = [_u8p, C.c_int, C.c_int, _u64p, _u8p]
+        L.orc_psnr_logical.restype = C.c_double
+        L.orc_psnr_physical.argtypes = [_u8p, C.c_int, C.c_int, _u64p]
+        L.orc_psnr_physical.restype = C.c_double
+        L.orc_synth_frame.argtypes = [C.POINTER(SynthCfg), C.c_int, _u8p]
+        L.orc_synth_frame.restype = None
+        L.orc_init()
+        self.L = L
+
+    # --- blocks ---------------------------------------------------------
+    def fit_block(self, px: np.ndarray) -> int:
+        px = np.ascontiguousarray(px, dtype=np.uint8).reshape(48)
+        return int(self.L.orc_fit_block(_p(px, _u8p)))
+
+    def fit_frame(self, rgb: np.ndarray) -> np.ndarray:
+        h, w, _ = rgb.shape
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+        out = np.zeros(((h + 3) // 4) * ((w + 3) // 4), dtype=np.uint64)
+        self.L.orc_fit_frame(_p(rgb, _u8p), w, h, _p(out, _u64p))
+        return out
+
+    def to_five_bits(self, x: int) -> int:
+        return int(self.L.orc_to_five_bits(x))
+
+    def to_six_bits(self, x: int) -> int:
+        return int(self.L.orc_to_six_bits(x))
+
+    def p2l(self, phys: int) -> np.ndarray:
+        out = np.zeros(40, dtype=np.uint8)
+        self.L.orc_p2l(C.c_uint64(phys), _p(out, _u8p))
+        return out
+
+    def l2p(self, logical: np.ndarray) -> int:
+        logical = np.ascontiguousarray(logical, dtype=np.uint8)
+        return int(self.L.orc_l2p(_p(logical, _u8p)))
+
+    def error(self, logical: np.ndarray, px: np.ndarray) -> int:
+        logical = np.ascontiguousarray(logical, dtype=np.uint8)
+        px = np.ascontiguousarray(px, dtype=np.uint8).reshape(48)
+        return int(self.L.orc_error(_p(logical, _u8p), _p(px, _u8p)))
+
+    def refit(self, logical: np.ndarray, px: np.ndarray) -> np.ndarray:
+        lb = np.array(logical, dtype=np.uint8)
+        px = np.ascontiguousarray(px, dtype=np.uint8).reshape(48)
+        self.L.orc_refit(_p(lb, _u8p), _p(px, _u8p))
+        return lb
+
+    def eval_candidate(self, own: int, px: np.ndarray, cand: int):
+        px = np.ascontiguousarray(px, dtype=np.uint8).reshape(48)
+        e, r = C.c_int(0), C.c_int(0)
+        eps = np.zeros(6, dtype=np.uint8)
+        ok = self.L.orc_eval_candidate(C.c_uint64(own), _p(px, _u8p), C.c_uint32(cand),
+                                       C.byref(e), C.byref(r), _p(eps, _u8p))
+        return (bool(ok), e.value, r.value, eps) if ok else (False, 0, 0, None)
+
+    # --- frames ---------------------------------------------------------
+    def reencode(self, rgb, phys_fit, S, thr, is_intra, ref_phys=None):
+        h, w, _ = rgb.shape
+        nb = ((h + 3) // 4) * ((w + 3) // 4)
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+        phys = np.array(phys_fit, dtype=np.uint64)
+        ref = np.zeros(nb, np.uint64) if ref_phys is None else np.ascontiguousarray(ref_phys, np.uint64)
+        motion = np.zeros(2 * nb, np.uint8)
+        unique = np.zeros(nb, np.uint32)
+        kind = np.zeros(nb, np.uint8)
+        eps = np.zeros(6 * nb, np.uint8)
+        n = self.L.orc_reencode(_p(rgb, _u8p), w, h, S, thr, int(bool(is_intra)), _p(ref, _u64p),
+                                _p(phys, _u64p), _p(motion, _u8p), _p(unique, _u32p),
+                                _p(kind, _u8p), _p(eps, _u8p))
+        return dict(phys=phys, motion=motion, unique=unique[:n].copy(), kind=kind, eps=eps)
+
+    def encode_sequence(self, frames, S, thr, intra_interval):
+        """The reference frame schedule (codec/codec.cpp:1379-1508) on the oracle."""
+        out = []
+        prev = None
+        set_intra, num = True, 1
+        for rgb in frames:
+            fit = self.fit_frame(rgb)
+            r = self.reencode(rgb, fit, S, thr, set_intra, None if set_intra else prev)
+            r["is_intra"] = set_intra
+            out.append(r)
+            prev = r["phys"]
+            set_intra, num = next_intra(set_intra, num, intra_interval)
+        return out
+
+    def psnr_logical(self, rgb, phys, eps) -> float:
+        h, w, _ = rgb.shape
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        phys = np.ascontiguousarray(phys, np.uint64)
+        eps = np.ascontiguousarray(eps, np.uint8)
+        return float(self.L.orc_psnr_logical(_p(rgb, _u8p), w, h, _p(phys, _u64p), _p(eps, _u8p)))
+
+    def psnr_physical(self, rgb, phys) -> float:
+        h, w, _ = rgb.shape
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        phys = np.ascontiguousarray(phys, np.uint64)
+        return float(self.L.orc_psnr_physical(_p(rgb, _u8p), w, h, _p(phys, _u64p)))
+
+    def synth_frame(self, cfg, t: int) -> np.ndarray:
+        cfg = SynthCfg(*[getattr(cfg, f[0]) for f in SynthCfg._fields_])
+        out = np.zeros((cfg.height, cfg.width, 3), dtype=np.uint8)
+        self.L.orc_synth_frame(C.byref(cfg), t, _p(out, _u8p))
+        return out
+
+    def synth_frames(self, cfg: SynthCfg, n: int) -> np.ndarray:
+        return np.stack([self.synth_frame(cfg, t) for t in range(n)])
+
+
+def next_intra(set_intra: bool, num: int, interval: int):
+    """Intra-frame schedule of codec/codec.cpp:1462-1469."""
+    if num <= interval:
+        set_intra = False
+        if num == interval:
+            set_intra, num = True, 1
+        else:
+            num += 1
+    return set_intra, num
+
+
+class Reference:
+    """The reference encoder compiled from its sources (build_ref.sh)."""
+
+    def __init__(self, path: str = REF_SO):
+        L = C.CDLL(path)
+        L.ref_fit_block.argtypes = [_u8p]
+        L.ref_fit_block.restype = C.c_uint64
+        L.ref_to_five_bits.argtypes = [C.c_uint8]
+        L.ref_to_five_bits.restype = C.c_uint8
+        L.ref_to_six_bits.argtypes = [C.c_uint8]
+        L.ref_to_six_bits.restype = C.c_uint8
+        L.ref_p2l.argtypes = [C.c_uint64, _u8p]
+        L.ref_p2l.restype = None
+        L.ref_l2p.argtypes = [_u8p]
+        L.ref_l2p.restype = C.c_uint64
+        L.ref_error.argtypes = [_u8p, _u8p]
+        L.ref_error.restype = C.c_int
+        L.ref_refit.argtypes = [_u8p, _u8p]
+        L.ref_refit.restype = None
+        L.ref_eval_candidate.argtypes = [C.c_uint64, _u8p, C.c_uint32, _i32p, _i32p, _u8p]
+        L.ref_eval_candidate.restype = C.c_int
+        L.ref_encode_sequence.argtypes = [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_int, C.c_char_p, _u64p, _u8p, _u32p, _i32p, _u8p,
+                                          _f64p, _f64p]
+        L.ref_encode_sequence.restype = C.c_int
+        L.ref_time_fit.argtypes = [_u8p, C.c_int, C.c_int, _u64p]
+        L.ref_time_fit.restype = C.c_double
+        self.L = L
+
+    def fit_block(self, px) -> int:
+        px = np.ascontiguousarray(px, dtype=np.uint8).reshape(48)
+        return int(self.L.ref_fit_block(_p(px, _u8p)))
+
+    def eval_candidate(self, own: int, px, cand: int):
+        px = np.ascontiguousarray(px, dtype=np.uint8).reshape(48)
+        e, r = C.c_int(0), C.c_int(0)
+        eps = np.zeros(6, dtype=np.uint8)
+        ok = self.L.ref_eval_candidate(C.c_uint64(own), _p(px, _u8p), C.c_uint32(cand),
+                                       C.byref(e), C.byref(r), _p(eps, _u8p))
+        return (bool(ok), e.value, r.value, eps) if ok else (False, 0, 0, None)
+
+    def encode_sequence(self, frames: np.ndarray, S: int, thr: int, intra_interval: int,
+                        tmpdir: str = "/tmp"):
+        frames = np.ascontiguousarray(frames, dtype=np.uint8)
+        n, h, w, _ = frames.shape
+        nb = ((h + 3) // 4) * ((w + 3) // 4)
+        phys = np.zeros(n * nb, np.uint64)
+        motion = np.zeros(n * nb * 2, np.uint8)
+        unique = np.zeros(n * nb, np.uint32)
+        n_unique = np.zeros(n, np.int32)
+        eps = np.zeros(n * nb * 6, np.uint8)
+        t_fit = np.zeros(n, np.float64)
+        t_re = np.zeros(n, np.float64)
+        d = os.path.join(tmpdir, f"mptc_ref_{os.getpid()}")
+        os.makedirs(d, exist_ok=True)
+        rc = self.L.ref_encode_sequence(_p(frames, _u8p), w, h, n, S, thr, intra_interval,
+                                        d.encode(), _p(phys, _u64p), _p(motion, _u8p),
+                                        _p(unique, _u32p), _p(n_unique, _i32p), _p(eps, _u8p),
+                                        _p(t_fit, _f64p), _p(t_re, _f64p))
+        try:
+            os.rmdir(d)
+        except OSError:
+            pass
+        if rc != 0:
+            raise RuntimeError(f"ref_encode_sequence failed rc={rc}")
+        out = []
+        for f in range(n):
+            out.append(dict(phys=phys[f * nb:(f + 1) * nb].copy(),
+                            motion=motion[f * 2 * nb:(f + 1) * 2 * nb].copy(),
+                            unique=unique[f * nb:f * nb + n_unique[f]].copy(),
+                            eps=eps[f * 6 * nb:(f + 1) * 6 * nb].copy(),
+                            t_fit=float(t_fit[f]), t_reencode=float(t_re[f])))
+        return out
+
+    def time_fit(self, rgb):
+        h, w, _ = rgb.shape
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        out = np.zeros(((h + 3) // 4) * ((w + 3) // 4), np.uint64)
+        t = self.L.ref_time_fit(_p(rgb, _u8p), w, h, _p(out, _u64p))
+        return float(t), out
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_SO)
