@@ -1,0 +1,121 @@
+/*
+ * mptc.h -- C-ABI of libmptc_hip.so, the MI355X (gfx950) implementation of
+ * MPTC's per-frame encode hot path: the DXT1 endpoint fit of every 4x4 block,
+ * the index-block reuse search over the inter and intra windows with its
+ * error threshold, and the least-squares endpoint re-fit of reused indices.
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every call returns 0
+ * (MPTC_OK) or a negative status; mptc_last_error() gives the message for the
+ * calling thread.  Nothing throws across the boundary.  A context owns one HIP
+ * stream and the device work buffers for frames up to its configured maximum
+ * size; use one context per host thread.  Output is bit-identical to the
+ * reference encoder for any launch configuration.
+ *
+ * Block layout (codec/dxt_image.h:16-23): one uint64_t per 4x4 block in raster
+ * order, bits 0-15 ep1 (RGB565), 16-31 ep2, 32-63 the 2-bit index word with
+ * pixel i = row*4 + col at bits 2i..2i+1.  Motion: two bytes (x, y) per block;
+ * inter reuse sets bit 7 of both, unique blocks are (255, 255)
+ * (codec/dxt_image.cpp:877, 892-894, 917, 953).
+ */
+#ifndef MPTC_H
+#define MPTC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MPTC_OK = 0,
+  MPTC_EINVAL = -1, /* bad argument: null pointer, size, search area, threshold */
+  MPTC_EHIP = -2,   /* HIP runtime error */
+  MPTC_ENOMEM = -3, /* device allocation failed */
+  MPTC_ESTATE = -4  /* call not valid in the context's current state */
+};
+
+typedef struct mptc_ctx mptc_ctx;
+
+typedef struct {
+  int32_t device;        /* HIP device ordinal */
+  int32_t max_width;     /* largest frame width in pixels (multiple of 4) */
+  int32_t max_height;    /* largest frame height in pixels (multiple of 4) */
+  int32_t search_area;   /* S in 1..63: window side 2S blocks */
+  int32_t err_threshold; /* reuse accepted iff error increase <= threshold,
+                            in floor(SSE/48) units, -32768..30000 */
+} mptc_config;
+
+/* Counters of the last encoded frame. */
+typedef struct {
+  int64_t n_blocks;
+  int64_t n_inter;          /* blocks reusing a previous-frame index word */
+  int64_t n_intra;          /* blocks reusing an index word of this frame */
+  int64_t n_unique;         /* blocks keeping their own index word */
+  int64_t inter_candidates; /* in-frame inter window positions searched */
+  int64_t intra_candidates; /* in-frame intra window positions of searched blocks */
+  int64_t reeval_candidates;/* intra positions re-evaluated after a word changed */
+  int32_t intra_iterations; /* passes until the intra words reached their fixpoint */
+  int32_t is_intra;
+} mptc_frame_stats;
+
+/* Context lifetime.  Replaces the process-global state the reference keeps in
+ * DXTImage's static members and vErrThreshold (codec/dxt_image.cpp:25,
+ * 357-360, 393-394). */
+int32_t mptc_create(const mptc_config* cfg, mptc_ctx** out);
+void mptc_destroy(mptc_ctx* ctx);
+const char* mptc_last_error(void);
+int32_t mptc_set_params(mptc_ctx* ctx, int32_t search_area, int32_t err_threshold);
+
+/* The DXTImage constructor's fit loop (codec/dxt_image.cpp:419-431,
+ * CompressRGB :123-149 -> stb_compress_dxt_block HIGHQUAL): host buffers.
+ * rgb: width*height*3 bytes; phys_out: (width/4)*(height/4) blocks. */
+int32_t mptc_fit(mptc_ctx* ctx, const uint8_t* rgb, int32_t width, int32_t height,
+                 uint64_t* phys_out);
+
+/* DXTImage::Reencode (codec/dxt_image.cpp:868-957): host buffers.
+ *  phys_inout  in: this frame's fitted blocks (as from mptc_fit); out: re-encoded
+ *  ref_phys    previous frame's re-encoded blocks; ignored (may be NULL) if is_intra
+ *  motion_out  2 bytes per block
+ *  unique_out  the unique palette (own index words of blocks not reused), raster order;
+ *              capacity one word per block; *n_unique_out receives the count
+ *  eps_out     optional (NULL): 6 bytes per block, logical ep1 rgb then ep2 rgb after
+ *              re-encode (the re-fit's unquantised-grid values, as _logical_blocks)
+ *  kind_out    optional (NULL): 0 unique, 1 intra reuse, 2 inter reuse */
+int32_t mptc_reencode(mptc_ctx* ctx, const uint8_t* rgb, int32_t width, int32_t height,
+                      int32_t is_intra, const uint64_t* ref_phys, uint64_t* phys_inout,
+                      uint8_t* motion_out, uint32_t* unique_out, uint32_t* n_unique_out,
+                      uint8_t* eps_out, uint8_t* kind_out);
+
+/* Fused, device-resident per-frame step: fit + Reencode of one frame whose
+ * RGB8 pixels are already in device memory.  All pointers are device
+ * pointers; d_ref_phys is the previous call's d_phys_out (ignored if is_intra).
+ * d_eps_out and d_kind_out may be NULL.  Work is queued on `hip_stream`
+ * (a hipStream_t; NULL = the context's stream); the call returns once the
+ * frame's work is queued and its intra fixpoint is reached (it synchronises
+ * with the stream to test convergence).  Same layouts as mptc_reencode. */
+int32_t mptc_encode_frame_device(mptc_ctx* ctx, const uint8_t* d_rgb, int32_t width,
+                                 int32_t height, int32_t is_intra, const uint64_t* d_ref_phys,
+                                 uint64_t* d_phys_out, uint8_t* d_motion_out,
+                                 uint32_t* d_unique_out, uint32_t* d_n_unique_out,
+                                 uint8_t* d_eps_out, uint8_t* d_kind_out, void* hip_stream);
+
+/* Counters of the last frame encoded with this context (synchronises). */
+int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
+
+/* Per-kernel timing with HIP events recorded on the launch stream.
+ * mptc_profile_enable(ctx, 1) starts recording; mptc_profile_read returns the
+ * accumulated launch count and milliseconds of kernel `name` ("fit", "table",
+ * "inter", "intra_dense", "intra_update", "intra_reeval", "intra_select",
+ * "commit", "unique") since it was enabled (synchronises). */
+int32_t mptc_profile_enable(mptc_ctx* ctx, int32_t enable);
+int32_t mptc_profile_read(mptc_ctx* ctx, const char* name, int64_t* launches, double* total_ms);
+
+/* Seeded synthetic RGB8 frames (see mptc_amd/csrc/framegen.h): writes
+ * n_frames frames starting at first_frame, each width*height*3 bytes.
+ * The layout of cfg is mptc_synth_cfg. */
+int32_t mptc_synth_frames(const void* cfg, int32_t first_frame, int32_t n_frames, uint8_t* rgb_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

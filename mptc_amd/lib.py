@@ -1,0 +1,203 @@
+"""ctypes binding of libmptc_hip.so (include/mptc.h).
+
+The library is built in-tree by mptc_amd/build.py; loading fails loudly if it
+is missing -- there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmptc_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "mptc.h")
+
+MPTC_OK, MPTC_EINVAL, MPTC_EHIP, MPTC_ENOMEM, MPTC_ESTATE = 0, -1, -2, -3, -4
+
+
+class MptcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mptc error {code}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [("device", C.c_int32), ("max_width", C.c_int32), ("max_height", C.c_int32),
+                ("search_area", C.c_int32), ("err_threshold", C.c_int32)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("n_blocks", C.c_int64), ("n_inter", C.c_int64), ("n_intra", C.c_int64),
+                ("n_unique", C.c_int64), ("inter_candidates", C.c_int64),
+                ("intra_candidates", C.c_int64), ("reeval_candidates", C.c_int64),
+                ("intra_iterations", C.c_int32), ("is_intra", C.c_int32)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+_u8p = C.POINTER(C.c_uint8)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_vp = C.c_void_p
+
+_lib = None
+
+
+def header_functions() -> list[str]:
+    """Names of the functions declared in include/mptc.h."""
+    text = open(HEADER).read()
+    return re.findall(r"^\s*(?:int32_t|void|const char\s*\*)\s*\**\s*(mptc_\w+)\s*\(", text, re.M)
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built (run python mptc_amd/build.py)")
+    L = C.CDLL(LIB_PATH)
+    L.mptc_last_error.restype = C.c_char_p
+    L.mptc_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+    L.mptc_destroy.argtypes = [_vp]
+    L.mptc_destroy.restype = None
+    L.mptc_set_params.argtypes = [_vp, C.c_int32, C.c_int32]
+    L.mptc_fit.argtypes = [_vp, _vp, C.c_int32, C.c_int32, _vp]
+    L.mptc_reencode.argtypes = [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp, _vp, _vp,
+                                _vp, _vp, _vp]
+    L.mptc_encode_frame_device.argtypes = [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp,
+                                           _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mptc_get_stats.argtypes = [_vp, C.POINTER(FrameStats)]
+    L.mptc_profile_enable.argtypes = [_vp, C.c_int32]
+    L.mptc_profile_read.argtypes = [_vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.mptc_synth_frames.argtypes = [_vp, C.c_int32, C.c_int32, _vp]
+    for name in header_functions():
+        if name not in ("mptc_destroy", "mptc_last_error"):
+            getattr(L, name).restype = C.c_int32
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != MPTC_OK:
+        raise MptcError(rc, load().mptc_last_error().decode())
+
+
+def _ptr(a) -> int | None:
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags.c_contiguous
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor (device or host)
+
+
+def synth_frames(cfg, first: int, n: int) -> np.ndarray:
+    out = np.zeros((n, cfg.height, cfg.width, 3), np.uint8)
+    _check(load().mptc_synth_frames(C.byref(cfg), first, n, out.ctypes.data))
+    return out
+
+
+class Encoder:
+    """One context: device work buffers for frames up to max_width x max_height."""
+
+    def __init__(self, max_width: int, max_height: int, search_area: int = 8,
+                 err_threshold: int = 0, device: int = 0):
+        L = load()
+        self.L = L
+        self.cfg = Config(device, max_width, max_height, search_area, err_threshold)
+        h = C.c_void_p()
+        _check(L.mptc_create(C.byref(self.cfg), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.mptc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, search_area: int, err_threshold: int):
+        _check(self.L.mptc_set_params(self.h, search_area, err_threshold))
+
+    # ---- host-buffer entry points (the DXTImage ctor fit loop and Reencode) ----
+    def fit(self, rgb: np.ndarray) -> np.ndarray:
+        h, w, _ = rgb.shape
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        out = np.zeros((h // 4) * (w // 4), np.uint64)
+        _check(self.L.mptc_fit(self.h, rgb.ctypes.data, w, h, out.ctypes.data))
+        return out
+
+    def reencode(self, rgb: np.ndarray, phys_fit: np.ndarray, is_intra: bool,
+                 ref_phys: np.ndarray | None = None):
+        h, w, _ = rgb.shape
+        nb = (h // 4) * (w // 4)
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        phys = np.array(phys_fit, dtype=np.uint64)
+        ref = None if ref_phys is None else np.ascontiguousarray(ref_phys, np.uint64)
+        motion = np.zeros(2 * nb, np.uint8)
+        unique = np.zeros(nb, np.uint32)
+        n_unique = np.zeros(1, np.uint32)
+        eps = np.zeros(6 * nb, np.uint8)
+        kind = np.zeros(nb, np.uint8)
+        _check(self.L.mptc_reencode(self.h, rgb.ctypes.data, w, h, int(bool(is_intra)),
+                                    None if ref is None else ref.ctypes.data, phys.ctypes.data,
+                                    motion.ctypes.data, unique.ctypes.data, n_unique.ctypes.data,
+                                    eps.ctypes.data, kind.ctypes.data))
+        return dict(phys=phys, motion=motion, unique=unique[:int(n_unique[0])].copy(), eps=eps,
+                    kind=kind)
+
+    # ---- device-resident step ----
+    def encode_frame_device(self, d_rgb, w: int, h: int, is_intra: bool, d_ref, d_phys,
+                            d_motion, d_unique, d_n_unique, d_eps=None, d_kind=None,
+                            stream: int | None = None):
+        _check(self.L.mptc_encode_frame_device(self.h, _ptr(d_rgb), w, h, int(bool(is_intra)),
+                                               _ptr(d_ref), _ptr(d_phys), _ptr(d_motion),
+                                               _ptr(d_unique), _ptr(d_n_unique), _ptr(d_eps),
+                                               _ptr(d_kind), stream))
+
+    def stats(self) -> dict:
+        s = FrameStats()
+        _check(self.L.mptc_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def profile_enable(self, on: bool = True):
+        _check(self.L.mptc_profile_enable(self.h, int(on)))
+
+    def profile_read(self, name: str):
+        n, ms = C.c_int64(), C.c_double()
+        _check(self.L.mptc_profile_read(self.h, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def encode_sequence(self, frames: np.ndarray, intra_interval: int):
+        """The reference frame schedule (codec/codec.cpp:1379-1508, entropy stage
+        excluded) on host buffers: fit, then Reencode against the previous frame."""
+        out = []
+        prev = None
+        set_intra, num = True, 1
+        for rgb in frames:
+            fit = self.fit(rgb)
+            r = self.reencode(rgb, fit, set_intra, None if set_intra else prev)
+            r["is_intra"] = set_intra
+            out.append(r)
+            prev = r["phys"]
+            set_intra, num = next_intra(set_intra, num, intra_interval)
+        return out
+
+
+def next_intra(set_intra: bool, num: int, interval: int):
+    """Intra-frame schedule of codec/codec.cpp:1462-1469."""
+    if num <= interval:
+        set_intra = False
+        if num == interval:
+            set_intra, num = True, 1
+        else:
+            num += 1
+    return set_intra, num

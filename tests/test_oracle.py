@@ -1,0 +1,148 @@
+"""The CPU oracle (oracle/mptc_oracle.c) pinned against the reference's own
+outputs (golden vectors from the compiled reference encoder) and against what
+the mathematics fixes independently of both."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from mptc_amd.workloads import SEQUENCES, synth_cfg
+from tests.helpers import compare_frame, load_seq, load_units
+
+
+@pytest.fixture(scope="module")
+def units():
+    return load_units()
+
+
+def test_fit_matches_reference_vectors(orc, units):
+    px, want = units["fit_px"], units["fit_out"]
+    got = np.array([orc.fit_block(p) for p in px], dtype=np.uint64)
+    assert np.array_equal(got, want)
+
+
+def test_snap_tables_match_reference(orc, units):
+    assert [orc.to_five_bits(x) for x in range(256)] == list(units["to_five"])
+    assert [orc.to_six_bits(x) for x in range(256)] == list(units["to_six"])
+
+
+def test_snap_is_a_grid_value_near_input(orc):
+    # every snapped value keeps the 5/6-bit grid point of one of its three
+    # candidates and never moves by more than a grid step
+    for x in range(256):
+        f, s = orc.to_five_bits(x), orc.to_six_bits(x)
+        assert abs(f - x) <= 8 and abs(s - x) <= 4
+        assert (f & 0xF8) in {(x & 0xF8), (x & 0xF8) + 4 & 0xF8, max((x & 0xF8) - 4, 0) & 0xF8}
+
+
+def test_candidate_eval_matches_reference_vectors(orc, units):
+    n = len(units["cand_word"])
+    for i in range(n):
+        ok, e, r, eps = orc.eval_candidate(int(units["cand_own"][i]), units["cand_px"][i],
+                                           int(units["cand_word"][i]))
+        assert ok == bool(units["cand_valid"][i]), i
+        if ok:
+            assert e == units["cand_err"][i], i
+            assert r == units["cand_reassigned"][i], i
+            assert np.array_equal(eps, units["cand_eps"][i]), i
+
+
+def _decode565(v):
+    r, g, b = (v >> 11) & 31, (v >> 5) & 63, v & 31
+    return np.array([(r << 3) | (r >> 2), (g << 2) | (g >> 4), (b << 3) | (b >> 2)], np.int64)
+
+
+def test_p2l_l2p_round_trip(orc):
+    # codec/dxt_image.cpp:236 asserts LogicalToPhysical(PhysicalToLogical(b)) == b
+    rng = np.random.default_rng(7)
+    words = rng.integers(0, 2**63, 20000, dtype=np.uint64) | (rng.integers(0, 2, 20000, dtype=np.uint64) << np.uint64(63))
+    for p in words[:20000]:
+        assert orc.l2p(orc.p2l(int(p))) == int(p)
+
+
+def test_error_closed_form(orc):
+    # Error = floor(sum over pixels and channels of (p - palette[idx])^2 / 48)
+    rng = np.random.default_rng(8)
+    for _ in range(2000):
+        px = rng.integers(0, 256, 48).astype(np.uint8)
+        e1, e2 = int(rng.integers(0, 65536)), int(rng.integers(0, 65536))
+        c = int(rng.integers(0, 2**32))
+        phys = e1 | (e2 << 16) | (c << 32)
+        a, b = _decode565(e1), _decode565(e2)
+        if e1 <= e2:
+            pal = [a, b, (a + b) // 2, np.zeros(3, np.int64)]
+        else:
+            pal = [a, b, (2 * a + b) // 3, (a + 2 * b) // 3]
+        sse = 0
+        for i in range(16):
+            q = pal[(c >> (2 * i)) & 3]
+            sse += int(((px[3 * i:3 * i + 3].astype(np.int64) - q) ** 2).sum())
+        assert orc.error(orc.p2l(phys), px) == sse // 48
+
+
+def test_single_index_candidates_are_invalid(orc):
+    # all 16 pixels on one index: the normal matrix is singular, 1/0 -> inf,
+    # inf*0 -> NaN, x86 truncation -> INT_MIN -> clamped 0: both endpoints 0,
+    # Pack565 equal, so the candidate can never keep its indices
+    rng = np.random.default_rng(9)
+    for _ in range(500):
+        px = rng.integers(0, 256, 48).astype(np.uint8)
+        own = orc.fit_block(px)
+        for c in (0x00000000, 0x55555555, 0xAAAAAAAA, 0xFFFFFFFF):
+            if c == own >> 32:
+                continue
+            ok, *_ = orc.eval_candidate(own, px, c)
+            assert not ok
+
+
+def test_own_word_is_free(orc):
+    rng = np.random.default_rng(10)
+    for _ in range(500):
+        px = rng.integers(0, 256, 48).astype(np.uint8)
+        own = orc.fit_block(px)
+        ok, e, r, _ = orc.eval_candidate(own, px, own >> 32)
+        assert ok and e == 0 and r == 0
+
+
+def test_refit_valid_candidate_is_four_colour_and_keeps_indices(orc):
+    # a valid re-fit packs with ep1 > ep2 and reproduces the candidate word
+    rng = np.random.default_rng(11)
+    n_valid = 0
+    for _ in range(3000):
+        px = np.clip(np.tile(rng.integers(0, 256, 3), 16) + rng.integers(-30, 31, 48), 0, 255).astype(np.uint8)
+        own = orc.fit_block(px)
+        c = int(rng.integers(0, 2**32))
+        ok, e, r, eps = orc.eval_candidate(own, px, c)
+        if not ok:
+            continue
+        n_valid += 1
+        lb = orc.p2l(own)
+        lb[32:40] = 0
+        for i in range(16):
+            lb[24 + i] = (c >> (2 * i)) & 3
+        lb = orc.refit(lb, px)
+        assert list(lb[0:3]) == list(eps[:3]) and list(lb[4:7]) == list(eps[3:])
+        p = orc.l2p(lb)
+        assert (p >> 32) == c and (p & 0xFFFF) > ((p >> 16) & 0xFFFF)
+    assert n_valid > 100
+
+
+@pytest.mark.parametrize("name", ["c1_s8_t0", "c1_s8_t8", "c1_s16_t0"])
+def test_sequence_matches_reference(orc, name):
+    fx = load_seq(name)
+    spec = SEQUENCES[name]
+    frames = orc.synth_frames(synth_cfg(spec), spec["golden_frames"])
+    res = orc.encode_sequence(frames, spec["S"], spec["thr"], spec["intra_interval"])
+    for f, (r, want) in enumerate(zip(res, fx["frames"])):
+        compare_frame(r, want, f"{name} frame {f}")
+
+
+def test_generator_is_stable(orc):
+    # the fixtures were made from these exact frames
+    for name in ("c1_s8_t0", "c1_s16_t0"):
+        fx = load_seq(name)
+        spec = SEQUENCES[name]
+        frames = orc.synth_frames(synth_cfg(spec), 2)
+        from tests.helpers import sha
+        assert sha(frames[0]) == fx["frames"][0]["input_sha"]
+        assert sha(frames[1]) == fx["frames"][1]["input_sha"]
