@@ -125,23 +125,31 @@ def main():
     # warmup
     run(a.warmup)
     stream.synchronize()
-    # timed region (per-kernel HIP events recorded on the launch stream)
-    enc.profile_enable(True)
+    # timed region: K steps bracketed by device synchronisation
     hip.synchronize()
     t0 = time.perf_counter()
     run(a.steps)
     stream.synchronize()
     hip.synchronize()
     t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = a.steps * nb / elapsed / 1e6
+    # the same K steps again with a HIP event pair around every kernel launch
+    # (recorded on the launch stream) for the per-kernel durations
+    enc.profile_enable(True)
+    hip.synchronize()
+    tp0 = time.perf_counter()
+    run(a.steps)
+    stream.synchronize()
+    hip.synchronize()
+    tp1 = time.perf_counter()
     kernels = {}
-    for name in ("fit", "table", "inter", "intra_dense", "intra_update", "intra_reeval", "intra_select",
-                 "commit", "unique", "zero"):
+    for name in ("fit", "table", "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine",
+                 "intra_apply", "intra_reeval", "intra_select", "commit", "unique", "zero"):
         n, ms = enc.profile_read(name)
         if n:
             kernels[name] = {"launches": n, "total_ms": round(ms, 4), "avg_ms": round(ms / n, 5)}
     enc.profile_enable(False)
-    elapsed = t1 - t0
-    value = a.steps * nb / elapsed / 1e6
 
     # work counters (one untimed pass over the same frames)
     stats = run(a.steps, collect=True)
@@ -186,6 +194,7 @@ def main():
                    "blocks_per_frame": nb, "device": hip.device_name(0)},
         "roofline": roof,
         "kernels": kernels,
+        "ms_per_step_with_kernel_events": round((tp1 - tp0) / a.steps * 1e3, 4),
         "work": {"inter_candidates": inter_c, "intra_candidates": intra_c, "reeval_candidates": reev_c,
                  **kinds, "max_intra_iterations": max(iters) if iters else 0},
         "psnr_db_last_frame": psnr,

@@ -37,13 +37,42 @@ enum {
 typedef struct mptc_ctx mptc_ctx;
 
 typedef struct {
-  int32_t device;        /* HIP device ordinal */
-  int32_t max_width;     /* largest frame width in pixels (multiple of 4) */
-  int32_t max_height;    /* largest frame height in pixels (multiple of 4) */
-  int32_t search_area;   /* S in 1..63: window side 2S blocks */
-  int32_t err_threshold; /* reuse accepted iff error increase <= threshold,
-                            in floor(SSE/48) units, -32768..30000 */
+  int32_t device;           /* HIP device ordinal */
+  int32_t max_width;        /* largest frame width in pixels (multiple of 4) */
+  int32_t max_height;       /* largest frame height in pixels (multiple of 4) */
+  int32_t search_area;      /* S in 1..63: window side 2S blocks */
+  int32_t err_threshold;    /* reuse accepted iff error increase <= threshold,
+                               in floor(SSE/48) units, -32768..30000 */
+  int32_t max_batch_frames; /* frames per mptc_encode_batch_device call, 1..64 (0 = 1) */
+  int32_t max_batch_blocks; /* 4x4 blocks summed over a batch
+                               (0 = max_batch_frames * (max_width/4) * (max_height/4)) */
 } mptc_config;
+
+/* Launch shape of the step, chosen by mptc_autotune or set directly.  Every
+ * setting gives bit-identical output. */
+typedef struct {
+  int32_t wg_size;    /* workgroup size of the per-block kernels: 64, 128, 256 */
+  int32_t tile_w;     /* blocks per wave row in the search kernels: 64, 32, 16, 8
+                         (the wave covers tile_w x 64/tile_w blocks) */
+  int32_t pass_batch; /* intra fixpoint passes queued per convergence check, 1..16 */
+  int32_t use_graph;  /* replay the step as captured HIP graphs */
+  int32_t win_chunks; /* window tile: the 2S window rows of a block are split into
+                         this many runs scanned by separate waves, 1..8 */
+} mptc_tuning;
+
+/* One frame of a batched step: device pointers, layouts as mptc_reencode. */
+typedef struct {
+  const uint8_t* d_rgb;
+  int32_t width, height;
+  int32_t is_intra;
+  const uint64_t* d_ref_phys; /* this stream's previous output (ignored if is_intra) */
+  uint64_t* d_phys_out;
+  uint8_t* d_motion_out;
+  uint32_t* d_unique_out;
+  uint32_t* d_n_unique_out;
+  uint8_t* d_eps_out;         /* optional */
+  uint8_t* d_kind_out;        /* optional */
+} mptc_frame_io;
 
 /* Counters of the last encoded frame. */
 typedef struct {
@@ -99,14 +128,36 @@ int32_t mptc_encode_frame_device(mptc_ctx* ctx, const uint8_t* d_rgb, int32_t wi
                                  uint32_t* d_unique_out, uint32_t* d_n_unique_out,
                                  uint8_t* d_eps_out, uint8_t* d_kind_out, void* hip_stream);
 
-/* Counters of the last frame encoded with this context (synchronises). */
+/* One step of several independent streams (uneven sizes allowed): frame i of
+ * the batch is encoded against frames[i].d_ref_phys exactly as
+ * mptc_encode_frame_device would, but all frames share every kernel launch
+ * of the step.  No reference counterpart (the reference encodes one stream per
+ * process); each frame's output equals DXTImage::Reencode on that frame. */
+int32_t mptc_encode_batch_device(mptc_ctx* ctx, int32_t n_frames, const mptc_frame_io* frames,
+                                 void* hip_stream);
+
+/* Launch-shape tuning.  mptc_autotune times every candidate shape on `steps`
+ * encodes of the given batch (outputs are overwritten), keeps the fastest and
+ * reports it. */
+int32_t mptc_get_tuning(mptc_ctx* ctx, mptc_tuning* out);
+int32_t mptc_set_tuning(mptc_ctx* ctx, const mptc_tuning* t);
+int32_t mptc_autotune(mptc_ctx* ctx, int32_t n_frames, const mptc_frame_io* frames, int32_t steps,
+                      mptc_tuning* best, double* best_ms_per_step);
+
+/* Waits for all work queued on the context's own stream (the stream used when
+ * a device entry point is given hip_stream = NULL). */
+int32_t mptc_synchronize(mptc_ctx* ctx);
+
+/* Counters of the last step encoded with this context, summed over its
+ * frames (synchronises). */
 int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
 
 /* Per-kernel timing with HIP events recorded on the launch stream.
  * mptc_profile_enable(ctx, 1) starts recording; mptc_profile_read returns the
  * accumulated launch count and milliseconds of kernel `name` ("fit", "table",
- * "inter", "intra_dense", "intra_update", "intra_reeval", "intra_select",
- * "commit", "unique") since it was enabled (synchronises). */
+ * "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine",
+ * "intra_apply", "intra_reeval", "intra_select", "commit", "unique", "zero") since it was enabled
+ * (synchronises).  While enabled, steps run as individual launches, not graphs. */
 int32_t mptc_profile_enable(mptc_ctx* ctx, int32_t enable);
 int32_t mptc_profile_read(mptc_ctx* ctx, const char* name, int64_t* launches, double* total_ms);
 

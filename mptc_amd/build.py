@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmptc_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "mptc_hip.hip")]
-HEADERS = [os.path.join(HERE, "csrc", f) for f in ("dxt_device.h", "framegen.h")] + [
+HEADERS = [os.path.join(HERE, "csrc", f) for f in ("dxt_device.h", "batch_kernels.h", "framegen.h")] + [
     os.path.join(ROOT, "include", "mptc.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -350,31 +350,57 @@ __device__ __forceinline__ uint32_t quant255(float p) {
   return (uint32_t)cl;
 }
 
+// quant255 when |p| < 2^31 is known: clamp then truncate.
+__device__ __forceinline__ uint32_t quant255_small(float p) {
+  return (uint32_t)__builtin_amdgcn_fmed3f(p + 0.5f, 0.0f, 255.0f);
+}
+
+// Candidates whose |f| is at most this produce |p| < 2^31 for any block
+// (|ax|, |bx| <= 16*255 and asq, bsq, |ab| <= 16), so the x86 overflow case
+// of quant255 cannot occur.
+constexpr float kSmallF = 16384.0f;
+
 // Per-block constants used by the candidate evaluation.
 struct BlockEval {
-  float f[48];        // pixel channels as float
-  uint32_t pk[16];    // pixel i packed r | g<<8 | b<<16
-  uint32_t sq;        // sum of squared channel values
-  uint32_t own;       // own (stb) index word
-  int orig;           // Error() of the own block
+  float f[48];       // pixel channels as float, pixel-major
+  uint32_t pc[12];   // pc[3j + ch]: channel ch of pixels 4j..4j+3, one byte each
+  uint32_t sq;       // sum of squared channel values
+  uint32_t own;      // own (stb) index word
+  int orig;          // Error() of the own block
 };
 
-__device__ __forceinline__ void make_block_eval(const BlockPx& px, uint64_t own_phys, BlockEval& be) {
-  uint32_t sq = 0;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const uint32_t r = px.ch(i, 0), g = px.ch(i, 1), b = px.ch(i, 2);
-    be.f[3 * i + 0] = (float)r;
-    be.f[3 * i + 1] = (float)g;
-    be.f[3 * i + 2] = (float)b;
-    be.pk[i] = r | (g << 8) | (b << 16);
-    sq += r * r + g * g + b * b;
-  }
-  be.sq = sq;
-  be.own = (uint32_t)(own_phys >> 32);
+// Error() of the block's own (fitted) logical block (codec/dxt_image.cpp:664-668).
+__device__ __forceinline__ int own_error(const BlockPx& px, uint64_t own_phys) {
   uint32_t pal[4];
   phys_palette(own_phys, pal);
-  be.orig = (int)(block_sse(px, be.own, pal) / 48u);
+  return (int)(block_sse(px, (uint32_t)(own_phys >> 32), pal) / 48u);
+}
+
+// Pixel-side constants of a block; `orig` is own_error(px, own_phys).
+__device__ __forceinline__ void make_block_eval(const BlockPx& px, uint64_t own_phys, int orig, BlockEval& be) {
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    be.f[3 * i + 0] = (float)px.ch(i, 0);
+    be.f[3 * i + 1] = (float)px.ch(i, 1);
+    be.f[3 * i + 2] = (float)px.ch(i, 2);
+  }
+  uint32_t sq = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      const uint32_t v = px.ch(4 * j, ch) | (px.ch(4 * j + 1, ch) << 8) | (px.ch(4 * j + 2, ch) << 16) |
+                         (px.ch(4 * j + 3, ch) << 24);
+      be.pc[3 * j + ch] = v;
+      sq = __builtin_amdgcn_udot4(v, v, sq, false);
+    }
+  be.sq = sq;
+  be.own = (uint32_t)(own_phys >> 32);
+  be.orig = orig;
+}
+
+__device__ __forceinline__ void make_block_eval(const BlockPx& px, uint64_t own_phys, BlockEval& be) {
+  make_block_eval(px, own_phys, own_error(px, own_phys), be);
 }
 
 // Result of one candidate: valid flag, error delta, quirky re-fit endpoints.
@@ -384,12 +410,35 @@ struct CandResult {
   uint32_t e1, e2;   // packed r | g<<8 | b<<16 quirky logical endpoints
 };
 
+// Spreads the 2-bit indices of pixels 4j..4j+3 into one byte each.
+__device__ __forceinline__ uint32_t spread_indices(uint32_t c, int j) {
+  const uint32_t b = (c >> (8 * j)) & 0xFFu;
+  const uint32_t x = (b | (b << 12)) & 0x000F000Fu;
+  return (x | (x << 6)) & 0x03030303u;
+}
+
+// Byte gather: byte n of the result is byte sel.byte[n] (0..3) of `table`.
+__device__ __forceinline__ uint32_t byte_gather(uint32_t table, uint32_t sel) {
+  return __builtin_amdgcn_perm(0u, table, sel);
+}
+
 // The body of the search loops (codec/dxt_image.cpp:677-698): AssignIndices,
 // RecalculateEndpoints (:290-351), the validity check (:689-694) and Error.
 // `snap` is the 512-byte snap table (LDS copy).
+// LSQ weights per 2-bit index, addressed by index*8 in LDS: (a, b) =
+// ((3-order)/3, order/3) with order = {0,3,1,2}[idx] (codec/dxt_image.cpp:301-308).
+__device__ __forceinline__ void init_ab_table(float2* s_ab) {
+  if (threadIdx.x < 4) {
+    const uint32_t k = threadIdx.x;
+    const uint32_t wa = k == 0 ? 3u : k == 1 ? 0u : k == 2 ? 2u : 1u;
+    s_ab[k] = make_float2((float)wa * kThird, (float)(3u - wa) * kThird);
+  }
+}
+
 template <bool kWantEps>
 __device__ __forceinline__ CandResult eval_candidate(const BlockEval& be, uint32_t c, const CandTab& t,
-                                                     const uint8_t* __restrict__ snap) {
+                                                     const uint8_t* __restrict__ snap,
+                                                     const float2* __restrict__ s_ab) {
   CandResult res;
   res.e1 = res.e2 = 0;
   if (c == be.own) {  // AssignIndices leaves the logical block unchanged
@@ -397,56 +446,71 @@ __device__ __forceinline__ CandResult eval_candidate(const BlockEval& be, uint32
     res.err = 0;
     return res;
   }
-  // ---- least squares sums over the 16 pixels, in pixel order ----
-  // wa = 3*a per pixel: idx 0->3, 1->0, 2->2, 3->1, i.e. bit1 = ~k0, bit0 = ~(k0^k1)
-  const uint32_t lo = c & 0x55555555u, hi = (c >> 1) & 0x55555555u;
-  const uint32_t wa = ((~lo & 0x55555555u) << 1) | (~(lo ^ hi) & 0x55555555u);
-  const uint32_t wb = ~wa;  // 3 - wa per 2-bit field
-  float ax0 = 0.0f, ax1 = 0.0f, ax2 = 0.0f, bx0 = 0.0f, bx1 = 0.0f, bx2 = 0.0f;
+  uint32_t sel[4];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const float a = (float)((wa >> (2 * i)) & 3u) * kThird;
-    const float b = (float)((wb >> (2 * i)) & 3u) * kThird;
-    ax0 += be.f[3 * i + 0] * a;
-    ax1 += be.f[3 * i + 1] * a;
-    ax2 += be.f[3 * i + 2] * a;
-    bx0 += be.f[3 * i + 0] * b;
-    bx1 += be.f[3 * i + 1] * b;
-    bx2 += be.f[3 * i + 2] * b;
+  for (int j = 0; j < 4; ++j) sel[j] = spread_indices(c, j);
+  // ---- least squares sums over the 16 pixels, in pixel order ----
+  // 0 + x == x for the non-negative products, so the sums start at pixel 0.
+  const char* ab_base = reinterpret_cast<const char*>(s_ab);
+  float ax0, ax1, ax2, bx0, bx1, bx2;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t s8 = sel[j] << 3;  // byte offsets of the (a, b) entries
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int i = 4 * j + n;
+      const float2 w = *reinterpret_cast<const float2*>(ab_base + ((s8 >> (8 * n)) & 0xFFu));
+      const float p0 = be.f[3 * i + 0], p1 = be.f[3 * i + 1], p2 = be.f[3 * i + 2];
+      if (i == 0) {
+        ax0 = p0 * w.x; ax1 = p1 * w.x; ax2 = p2 * w.x;
+        bx0 = p0 * w.y; bx1 = p1 * w.y; bx2 = p2 * w.y;
+      } else {
+        ax0 += p0 * w.x; ax1 += p1 * w.x; ax2 += p2 * w.x;
+        bx0 += p0 * w.y; bx1 += p1 * w.y; bx2 += p2 * w.y;
+      }
+    }
   }
   // ---- solve, quantise, snap ----
-  const uint32_t r1 = snap[quant255(t.f * (ax0 * t.bsq - bx0 * t.ab))];
-  const uint32_t g1 = snap[256 + quant255(t.f * (ax1 * t.bsq - bx1 * t.ab))];
-  const uint32_t b1 = snap[quant255(t.f * (ax2 * t.bsq - bx2 * t.ab))];
-  const uint32_t r2 = snap[quant255(t.f * (bx0 * t.asq - ax0 * t.ab))];
-  const uint32_t g2 = snap[256 + quant255(t.f * (bx1 * t.asq - ax1 * t.ab))];
-  const uint32_t b2 = snap[quant255(t.f * (bx2 * t.asq - ax2 * t.ab))];
+  const float s1r = t.f * (ax0 * t.bsq - bx0 * t.ab), s1g = t.f * (ax1 * t.bsq - bx1 * t.ab);
+  const float s1b = t.f * (ax2 * t.bsq - bx2 * t.ab), s2r = t.f * (bx0 * t.asq - ax0 * t.ab);
+  const float s2g = t.f * (bx1 * t.asq - ax1 * t.ab), s2b = t.f * (bx2 * t.asq - ax2 * t.ab);
+  uint32_t r1, g1, b1, r2, g2, b2;
+  if (fabsf(t.f) <= kSmallF) {
+    r1 = snap[quant255_small(s1r)]; g1 = snap[256 + quant255_small(s1g)]; b1 = snap[quant255_small(s1b)];
+    r2 = snap[quant255_small(s2r)]; g2 = snap[256 + quant255_small(s2g)]; b2 = snap[quant255_small(s2b)];
+  } else {
+    r1 = snap[quant255(s1r)]; g1 = snap[256 + quant255(s1g)]; b1 = snap[quant255(s1b)];
+    r2 = snap[quant255(s2r)]; g2 = snap[256 + quant255(s2g)]; b2 = snap[quant255(s2b)];
+  }
   // ---- validity: LogicalToPhysical keeps the indices iff Pack565(e1) > Pack565(e2) ----
   res.valid = pack565(r1, g1, b1) > pack565(r2, g2, b2);
   if (!res.valid) {
     res.err = 0;
     return res;
   }
-  // ---- palette (LerpChannels 1/3, 2/3) and error ----
-  uint32_t pal[4];
-  pal[0] = r1 | (g1 << 8) | (b1 << 16);
-  pal[1] = r2 | (g2 << 8) | (b2 << 16);
-  pal[2] = div3(2 * r1 + r2) | (div3(2 * g1 + g2) << 8) | (div3(2 * b1 + b2) << 16);
-  pal[3] = div3(r1 + 2 * r2) | (div3(g1 + 2 * g2) << 8) | (div3(b1 + 2 * b2) << 16);
+  // ---- palette (LerpChannels 1/3, 2/3), one byte per entry per channel ----
+  const uint32_t pr = r1 | (r2 << 8) | (div3(2 * r1 + r2) << 16) | (div3(r1 + 2 * r2) << 24);
+  const uint32_t pg = g1 | (g2 << 8) | (div3(2 * g1 + g2) << 16) | (div3(g1 + 2 * g2) << 24);
+  const uint32_t pb = b1 | (b2 << 8) | (div3(2 * b1 + b2) << 16) | (div3(b1 + 2 * b2) << 24);
   // SSE = sum|p|^2 - 2 sum p.q + sum|q|^2, all exact in 32-bit integers
   uint32_t dot = 0, qq = 0;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const uint32_t k = (c >> (2 * i)) & 3u;
-    const uint32_t q = (k & 2u) ? ((k & 1u) ? pal[3] : pal[2]) : ((k & 1u) ? pal[1] : pal[0]);
-    dot = __builtin_amdgcn_udot4(be.pk[i], q, dot, false);
-    qq = __builtin_amdgcn_udot4(q, q, qq, false);
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t qr = byte_gather(pr, sel[j]);
+    const uint32_t qg = byte_gather(pg, sel[j]);
+    const uint32_t qb = byte_gather(pb, sel[j]);
+    dot = __builtin_amdgcn_udot4(be.pc[3 * j + 0], qr, dot, false);
+    dot = __builtin_amdgcn_udot4(be.pc[3 * j + 1], qg, dot, false);
+    dot = __builtin_amdgcn_udot4(be.pc[3 * j + 2], qb, dot, false);
+    qq = __builtin_amdgcn_udot4(qr, qr, qq, false);
+    qq = __builtin_amdgcn_udot4(qg, qg, qq, false);
+    qq = __builtin_amdgcn_udot4(qb, qb, qq, false);
   }
   const uint32_t sse = be.sq - 2u * dot + qq;
   res.err = (int)(sse / 48u) - be.orig;
   if (kWantEps) {
-    res.e1 = pal[0];
-    res.e2 = pal[1];
+    res.e1 = r1 | (g1 << 8) | (b1 << 16);
+    res.e2 = r2 | (g2 << 8) | (b2 << 16);
   }
   return res;
 }

@@ -1,446 +1,36 @@
-// libmptc_hip.so: HIP kernels for gfx950 and the C-ABI of include/mptc.h.
+// libmptc_hip.so: HIP kernels for gfx950 (batch_kernels.h, dxt_device.h) and
+// the C-ABI of include/mptc.h.
 //
-// Per frame (device-resident):
-//   k_fit          stb DXT1 fit of every block (lane per block)
-//   k_table        per-position candidate table of the reference frame's words
-//   k_inter        InterBlockSearch + accept test, lane per block, exact scan order
-//   k_intra_prep   which blocks need the intra search; initial current words
-//   k_intra_dense  every intra candidate of every such block (lane per block);
-//                  stores the error deltas E[pos][block] and folds the selection
-//   k_intra_update words of blocks whose decision changed -> change list
-//   k_intra_reeval re-evaluates the (block, position) pairs that see a changed word
-//   k_intra_select re-folds the selection of blocks with a re-evaluated position
-//   (update / reeval / select repeat until no word changes: the intra words are
-//    the unique fixpoint of a raster-triangular system, which equals the
-//    reference's sequential raster-order result)
-//   k_commit       final blocks, motion bytes, logical endpoints, unique flags
-//   k_unique       unique palette in raster order
+// One encode step = one frame of each of up to kMaxFrames streams.  The frames
+// are described in device memory, so every kernel of the step covers the whole
+// batch in one launch, and the step's launch sequence is the same for every
+// step of a given batch shape: it is captured once as three HIP graphs
+// (prefix: counters, fit, reference tables, inter search, intra set-up and
+// dense pass; one batch of fixpoint passes; suffix: commit and unique
+// palettes) and replayed.  Between batches of passes the host reads one
+// counter to learn whether the intra words reached their fixpoint; one batch
+// stays queued while it does.
 #include <hip/hip_runtime.h>
 
-#include <atomic>
+#include <algorithm>
+#include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/mptc.h"
-#include "dxt_device.h"
+#include "batch_kernels.h"
 #include "framegen.h"
 
 using namespace mptc;
 
 namespace {
 
-constexpr int kWG = 256;          // threads per workgroup (4 wave64)
-constexpr int kUniqueTile = 1024; // blocks per unique-compaction tile
-constexpr int16_t kNoCand = 32767;// E sentinel: invalid or absent candidate
 constexpr int kMaxS = 63;
 
-// Per-block search decision, packed into 64 bits:
-// [0,32) index word, [32,40) motion x, [40,48) motion y, bit 48 accepted.
-__device__ __forceinline__ uint64_t pack_dec(uint32_t c, uint32_t x, uint32_t y, bool acc) {
-  return (uint64_t)c | ((uint64_t)x << 32) | ((uint64_t)y << 40) | ((uint64_t)(acc ? 1 : 0) << 48);
-}
-__device__ __forceinline__ bool dec_acc(uint64_t d) { return (d >> 48) & 1; }
-__device__ __forceinline__ uint32_t dec_word(uint64_t d) { return (uint32_t)d; }
-
-struct DevStats {
-  unsigned long long inter_cands, intra_cands, reeval_cands;
-  unsigned int n_inter, n_intra, n_unique, n_changed;
-};
-
-// Adds a per-lane count to a global counter with one atomic per workgroup.
-// Every thread of the workgroup must call it.
-__device__ __forceinline__ void wg_count(unsigned long long* s_acc, unsigned long long v,
-                                         unsigned long long* global) {
-  if (threadIdx.x == 0) *s_acc = 0;
-  __syncthreads();
-  if (v) atomicAdd(s_acc, v);
-  __syncthreads();
-  if (threadIdx.x == 0 && *s_acc) atomicAdd(global, *s_acc);
-}
-
-__device__ __forceinline__ void load_snap_lds(uint8_t* s_snap) {
-  for (int i = threadIdx.x; i < 512; i += blockDim.x) s_snap[i] = c_snap[i];
-  __syncthreads();
-}
-
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kWG) k_fit(const uint8_t* __restrict__ rgb, int width, int bw,
-                                             int nb, uint64_t* __restrict__ own) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nb) return;
-  BlockPx px;
-  load_block(rgb, width, k % bw, k / bw, px);
-  own[k] = fit_block(px);
-}
-
-// Candidate table of a word array (stride in 32-bit words: 2 reads the index
-// word of a uint64_t physical block array).
-__global__ void __launch_bounds__(kWG) k_table(const uint32_t* __restrict__ words, int stride, int nb,
-                                               uint32_t* __restrict__ tw, float4* __restrict__ tab) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nb) return;
-  const uint32_t c = words[(size_t)q * stride];
-  const CandTab t = cand_table(c);
-  tw[q] = c;
-  tab[q] = make_float4(t.asq, t.bsq, t.ab, t.f);
-}
-
-// InterBlockSearch (codec/dxt_image.cpp:715-774) + the accept test (:890).
-// Rows j = by-S .. by+S-1 ascending, columns i = bx-S .. bx+S-1 ascending;
-// the first error delta <= 0 clamps the running minimum to 0 and leaves the
-// column loop only.
-__global__ void __launch_bounds__(kWG) k_inter(const uint8_t* __restrict__ rgb, int width, int bw, int bh,
-                                               int S, int thr, const uint64_t* __restrict__ own,
-                                               const uint32_t* __restrict__ rw, const float4* __restrict__ rtab,
-                                               uint64_t* __restrict__ dec_out, DevStats* __restrict__ st) {
-  __shared__ uint8_t s_snap[512];
-  __shared__ unsigned long long s_acc;
-  load_snap_lds(s_snap);
-  const int nb = bw * bh;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long n_cand = 0;
-  if (k < nb) {
-    const int bx = k % bw, by = k / bw;
-    BlockPx px;
-    load_block(rgb, width, bx, by, px);
-    BlockEval be;
-    make_block_eval(px, own[k], be);
-    int min_err = 0x7fffffff;
-    uint32_t sel_c = 0, sel_x = 0, sel_y = 0;
-    const int i_lo = max(bx - S, 0), i_hi = min(bx + S, bw);
-    const int j_lo = max(by - S, 0), j_hi = min(by + S, bh);
-    for (int j = j_lo; j < j_hi; ++j) {
-      for (int i = i_lo; i < i_hi; ++i) {
-        const int q = j * bw + i;
-        const uint32_t c = rw[q];
-        const float4 t4 = rtab[q];
-        const CandResult r = eval_candidate<false>(be, c, CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap);
-        if (r.valid && r.err < min_err) {
-          min_err = r.err;
-          sel_c = c;
-          sel_x = (uint32_t)(i - bx + S);
-          sel_y = (uint32_t)(j - by + S);
-          if (r.err <= 0) {
-            min_err = 0;
-            break;  // leaves the column loop only
-          }
-        }
-      }
-    }
-    dec_out[k] = pack_dec(sel_c, sel_x | 0x80u, sel_y | 0x80u, min_err <= thr);
-    n_cand = (unsigned long long)((i_hi - i_lo) * (j_hi - j_lo));
-  }
-  wg_count(&s_acc, n_cand, &st->inter_cands);
-}
-
-// Blocks that need the intra search, and the current words the search sees
-// before any intra decision: inter-reused words are final (they read only the
-// previous frame), every other block starts from its own word.
-__global__ void __launch_bounds__(kWG) k_intra_prep(int nb, int is_intra, const uint64_t* __restrict__ own,
-                                                    const uint64_t* __restrict__ inter_dec,
-                                                    uint8_t* __restrict__ need, uint32_t* __restrict__ cw,
-                                                    float4* __restrict__ ctab, uint8_t* __restrict__ dirty) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nb) return;
-  bool inter_ok = false;
-  uint32_t w = (uint32_t)(own[q] >> 32);
-  if (!is_intra) {
-    const uint64_t d = inter_dec[q];
-    if (dec_acc(d)) {
-      inter_ok = true;
-      w = dec_word(d);
-    }
-  }
-  need[q] = inter_ok ? 0 : 1;
-  dirty[q] = 0;
-  cw[q] = w;
-  const CandTab t = cand_table(w);
-  ctab[q] = make_float4(t.asq, t.bsq, t.ab, t.f);
-}
-
-// Intra scan order (codec/dxt_image.cpp:672-675): rows j = by .. by-2S+1
-// (r = by - j ascending), columns i = bx+S-1 .. bx-S (t = bx+S-1-i ascending),
-// skipping (j == by, i >= bx).  Position pos = r*2S + t.
-__device__ __forceinline__ uint64_t fold_intra(const int16_t* __restrict__ E, size_t nbs, int k, int S) {
-  const int W2 = 2 * S;
-  int min_err = 0x7fffffff, sel = -1;
-  for (int r = 0; r < W2; ++r) {
-    for (int t = (r == 0 ? S : 0); t < W2; ++t) {
-      const int e = E[(size_t)(r * W2 + t) * nbs + k];
-      if (e == kNoCand) continue;
-      if (e < min_err) {
-        min_err = e;
-        sel = r * W2 + t;
-        if (e <= 0) {
-          min_err = 0;
-          break;
-        }
-      }
-    }
-  }
-  // the word is filled in by the caller from the current words
-  return (uint64_t)(uint32_t)sel | ((uint64_t)(uint32_t)min_err << 32);
-}
-
-__device__ __forceinline__ uint64_t intra_decision(const uint32_t* __restrict__ cw, int bw, int k, int S,
-                                                   int thr, uint64_t folded) {
-  const int sel = (int)(uint32_t)folded;
-  const int min_err = (int)(uint32_t)(folded >> 32);
-  if (sel < 0 || min_err > thr) return 0;
-  const int W2 = 2 * S;
-  const int r = sel / W2, t = sel % W2;
-  const int bx = k % bw, by = k / bw;
-  const int j = by - r, i = bx + S - 1 - t;
-  return pack_dec(cw[j * bw + i], (uint32_t)(i - bx + S), (uint32_t)(j - by + 2 * S - 1), true);
-}
-
-__global__ void __launch_bounds__(kWG) k_intra_dense(const uint8_t* __restrict__ rgb, int width, int bw, int bh,
-                                                     int S, int thr, const uint64_t* __restrict__ own,
-                                                     const uint8_t* __restrict__ need,
-                                                     const uint32_t* __restrict__ cw,
-                                                     const float4* __restrict__ ctab, int16_t* __restrict__ E,
-                                                     size_t nbs, uint64_t* __restrict__ dec,
-                                                     DevStats* __restrict__ st) {
-  __shared__ uint8_t s_snap[512];
-  __shared__ unsigned long long s_acc;
-  load_snap_lds(s_snap);
-  const int nb = bw * bh;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long n_cand = 0;
-  if (k < nb && need[k]) {
-    const int bx = k % bw, by = k / bw;
-    BlockPx px;
-    load_block(rgb, width, bx, by, px);
-    BlockEval be;
-    make_block_eval(px, own[k], be);
-    const int W2 = 2 * S;
-    for (int r = 0; r < W2; ++r) {
-      const int j = by - r;
-      for (int t = 0; t < W2; ++t) {
-        const int i = bx + S - 1 - t;
-        int16_t ev = kNoCand;
-        if (j >= 0 && i >= 0 && i < bw && !(r == 0 && t < S)) {
-          const int q = j * bw + i;
-          const float4 t4 = ctab[q];
-          const CandResult res = eval_candidate<false>(be, cw[q], CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap);
-          ++n_cand;
-          if (res.valid) ev = (int16_t)min(max(res.err, -32768), 32766);
-        }
-        E[(size_t)(r * W2 + t) * nbs + k] = ev;
-      }
-    }
-    dec[k] = intra_decision(cw, bw, k, S, thr, fold_intra(E, nbs, k, S));
-  }
-  wg_count(&s_acc, n_cand, &st->intra_cands);
-}
-
-// New current words from the decisions; changed positions go to the list.
-__global__ void __launch_bounds__(kWG) k_intra_update(int nb, const uint64_t* __restrict__ own,
-                                                      const uint8_t* __restrict__ need,
-                                                      const uint64_t* __restrict__ dec, uint32_t* __restrict__ cw,
-                                                      float4* __restrict__ ctab, int* __restrict__ chg_list,
-                                                      DevStats* __restrict__ st) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nb || !need[q]) return;
-  const uint64_t d = dec[q];
-  const uint32_t w = dec_acc(d) ? dec_word(d) : (uint32_t)(own[q] >> 32);
-  if (w != cw[q]) {
-    cw[q] = w;
-    const CandTab t = cand_table(w);
-    ctab[q] = make_float4(t.asq, t.bsq, t.ab, t.f);
-    const unsigned slot = atomicAdd(&st->n_changed, 1u);
-    chg_list[slot] = q;
-  }
-}
-
-// Every (block, position) pair whose intra window holds a changed position is
-// re-evaluated with the new word.  Work item = (changed position, slot).
-__global__ void __launch_bounds__(kWG) k_intra_reeval(const uint8_t* __restrict__ rgb, int width, int bw, int bh,
-                                                      int S, const uint64_t* __restrict__ own,
-                                                      const uint8_t* __restrict__ need,
-                                                      const uint32_t* __restrict__ cw,
-                                                      const float4* __restrict__ ctab,
-                                                      const int* __restrict__ chg_list, int16_t* __restrict__ E,
-                                                      size_t nbs, uint8_t* __restrict__ dirty,
-                                                      DevStats* __restrict__ st) {
-  __shared__ uint8_t s_snap[512];
-  __shared__ unsigned long long s_acc;
-  load_snap_lds(s_snap);
-  const int W2 = 2 * S, slots = W2 * W2;
-  const unsigned n_chg = st->n_changed;
-  const size_t total = (size_t)n_chg * slots;
-  unsigned long long n_eval = 0;
-  for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (size_t)gridDim.x * blockDim.x) {
-    const int q = chg_list[w / slots];
-    const int s = (int)(w % slots);
-    const int r = s / W2, t = s % W2;
-    if (r == 0 && t < S) continue;
-    const int qx = q % bw, qy = q / bw;
-    const int by = qy + r, bx = qx - S + 1 + t;
-    if (by >= bh || bx < 0 || bx >= bw) continue;
-    const int k = by * bw + bx;
-    if (!need[k]) continue;
-    BlockPx px;
-    load_block(rgb, width, bx, by, px);
-    BlockEval be;
-    make_block_eval(px, own[k], be);
-    const float4 t4 = ctab[q];
-    const CandResult res = eval_candidate<false>(be, cw[q], CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap);
-    E[(size_t)s * nbs + k] = res.valid ? (int16_t)min(max(res.err, -32768), 32766) : kNoCand;
-    dirty[k] = 1;
-    ++n_eval;
-  }
-  wg_count(&s_acc, n_eval, &st->reeval_cands);
-}
-
-__global__ void __launch_bounds__(kWG) k_intra_select(int bw, int bh, int S, int thr,
-                                                      const uint32_t* __restrict__ cw,
-                                                      const int16_t* __restrict__ E, size_t nbs,
-                                                      uint8_t* __restrict__ dirty, uint64_t* __restrict__ dec) {
-  const int nb = bw * bh;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nb || !dirty[k]) return;
-  dirty[k] = 0;
-  dec[k] = intra_decision(cw, bw, k, S, thr, fold_intra(E, nbs, k, S));
-}
-
-__global__ void k_zero_u32(uint32_t* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
-// Commit (codec/dxt_image.cpp:900-905, 922-926, 953-954).
-__global__ void __launch_bounds__(kWG) k_commit(const uint8_t* __restrict__ rgb, int width, int bw, int nb,
-                                                int is_intra, const uint64_t* __restrict__ own,
-                                                const uint64_t* __restrict__ inter_dec,
-                                                const uint8_t* __restrict__ need,
-                                                const uint64_t* __restrict__ intra_dec,
-                                                uint64_t* __restrict__ phys_out, uint8_t* __restrict__ motion,
-                                                uint8_t* __restrict__ eps, uint8_t* __restrict__ kind,
-                                                uint32_t* __restrict__ tile_count, DevStats* __restrict__ st) {
-  __shared__ uint8_t s_snap[512];
-  __shared__ unsigned long long s_acc;
-  load_snap_lds(s_snap);
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int kd = -1;
-  if (k < nb) {
-    const uint64_t o = own[k];
-    uint64_t d = 0;
-    kd = 0;
-    if (!is_intra && dec_acc(inter_dec[k])) {
-      d = inter_dec[k];
-      kd = 2;
-    } else if (need[k] && dec_acc(intra_dec[k])) {
-      d = intra_dec[k];
-      kd = 1;
-    }
-    uint64_t p = o;
-    uint32_t e1, e2;
-    {
-      uint32_t r1, g1, b1, r2, g2, b2;
-      decode565((uint32_t)(o & 0xFFFF), r1, g1, b1);
-      decode565((uint32_t)((o >> 16) & 0xFFFF), r2, g2, b2);
-      e1 = r1 | (g1 << 8) | (b1 << 16);
-      e2 = r2 | (g2 << 8) | (b2 << 16);
-    }
-    uint8_t mx = 255, my = 255;
-    if (kd) {
-      const uint32_t c = dec_word(d);
-      mx = (uint8_t)(d >> 32);
-      my = (uint8_t)(d >> 40);
-      if (c != (uint32_t)(o >> 32)) {
-        // re-assigned indices: RecalculateEndpoints (codec/dxt_image.cpp:901-905, 923-926)
-        BlockPx px;
-        load_block(rgb, width, k % bw, k / bw, px);
-        BlockEval be;
-        make_block_eval(px, o, be);
-        const CandResult r = eval_candidate<true>(be, c, cand_table(c), s_snap);
-        e1 = r.e1;
-        e2 = r.e2;
-        p = (uint64_t)pack565(e1 & 0xFF, (e1 >> 8) & 0xFF, e1 >> 16) |
-            ((uint64_t)pack565(e2 & 0xFF, (e2 >> 8) & 0xFF, e2 >> 16) << 16) | ((uint64_t)c << 32);
-      }
-    }
-    phys_out[k] = p;
-    motion[2 * k] = mx;
-    motion[2 * k + 1] = my;
-    if (eps) {
-      eps[6 * k + 0] = (uint8_t)e1; eps[6 * k + 1] = (uint8_t)(e1 >> 8); eps[6 * k + 2] = (uint8_t)(e1 >> 16);
-      eps[6 * k + 3] = (uint8_t)e2; eps[6 * k + 4] = (uint8_t)(e2 >> 8); eps[6 * k + 5] = (uint8_t)(e2 >> 16);
-    }
-    if (kind) kind[k] = (uint8_t)kd;
-  }
-  // one tile of kUniqueTile blocks spans kUniqueTile / kWG whole workgroups
-  __shared__ unsigned s_cnt[3];
-  if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
-  __syncthreads();
-  if (kd >= 0) atomicAdd(&s_cnt[kd], 1u);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (s_cnt[0]) atomicAdd(&tile_count[(blockIdx.x * blockDim.x) / kUniqueTile], s_cnt[0]);
-    if (s_cnt[1]) atomicAdd(&st->n_intra, s_cnt[1]);
-    if (s_cnt[2]) atomicAdd(&st->n_inter, s_cnt[2]);
-  }
-  (void)s_acc;
-}
-
-// Unique palette in raster order: workgroup per tile of kUniqueTile blocks.
-__global__ void __launch_bounds__(kWG) k_unique(int nb, const uint64_t* __restrict__ own,
-                                                const uint8_t* __restrict__ motion,
-                                                const uint32_t* __restrict__ tile_count, int n_tiles,
-                                                uint32_t* __restrict__ unique_out, uint32_t* __restrict__ n_unique,
-                                                DevStats* __restrict__ st) {
-  __shared__ uint32_t s_sum[kWG];
-  __shared__ uint32_t s_wave[kWG / 64];
-  const int tile = blockIdx.x;
-  // prefix of the preceding tiles
-  uint32_t acc = 0;
-  for (int t = threadIdx.x; t < tile; t += blockDim.x) acc += tile_count[t];
-  s_sum[threadIdx.x] = acc;
-  __syncthreads();
-  for (int off = kWG / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) s_sum[threadIdx.x] += s_sum[threadIdx.x + off];
-    __syncthreads();
-  }
-  uint32_t base = s_sum[0];
-  __syncthreads();
-  // each thread handles kUniqueTile / kWG consecutive blocks
-  constexpr int per = kUniqueTile / kWG;
-  const int k0 = tile * kUniqueTile + threadIdx.x * per;
-  uint32_t flags = 0, cnt = 0;
-#pragma unroll
-  for (int u = 0; u < per; ++u) {
-    const int k = k0 + u;
-    const bool uq = k < nb && motion[2 * k] == 255 && motion[2 * k + 1] == 255;
-    flags |= (uq ? 1u : 0u) << u;
-    cnt += uq ? 1u : 0u;
-  }
-  // exclusive scan of cnt across the workgroup
-  s_sum[threadIdx.x] = cnt;
-  __syncthreads();
-  for (int off = 1; off < kWG; off <<= 1) {
-    const uint32_t v = (int)threadIdx.x >= off ? s_sum[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s_sum[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t pos = base + s_sum[threadIdx.x] - cnt;
-#pragma unroll
-  for (int u = 0; u < per; ++u)
-    if (flags & (1u << u)) unique_out[pos++] = (uint32_t)(own[k0 + u] >> 32);
-  (void)s_wave;
-  if (tile == n_tiles - 1 && threadIdx.x == kWG - 1) {
-    *n_unique = pos;
-    st->n_unique = pos;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Host side
-// ---------------------------------------------------------------------------
 thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
@@ -448,11 +38,11 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                   \
-  do {                                                                                  \
-    hipError_t e_ = (expr);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail(MPTC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+#define HIP_TRY(expr)                                                             \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(MPTC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
 // stb__PrepareOptTable (Include/stb_dxt.h:111-137)
@@ -508,33 +98,65 @@ int init_tables(int device) {
   return MPTC_OK;
 }
 
-enum KernelId { KF_ZERO, KF_FIT, KF_TABLE, KF_INTER, KF_DENSE, KF_UPDATE, KF_REEVAL, KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_N };
-const char* kKernelNames[KF_N] = {"zero", "fit", "table", "inter", "intra_dense", "intra_update",
-                                  "intra_reeval", "intra_select", "commit", "unique"};
+enum KernelId {
+  KF_ZERO, KF_FIT, KF_TABLE, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
+  KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_N
+};
+const char* kKernelNames[KF_N] = {"zero",         "fit",          "table",       "inter",
+                                  "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
+                                  "intra_apply",  "intra_reeval", "intra_select", "commit",
+                                  "unique"};
+constexpr int kMaxChunks = 8;  // window-row chunks per block in the search kernels
+
+constexpr int kReevalGrid = 4096;  // workgroups of the re-evaluation pass (grid-stride)
+constexpr int kListGrid = 256;     // workgroups of the re-select pass (grid-stride)
+
+struct Graphs {
+  hipGraphExec_t prefix = nullptr, passes = nullptr, suffix = nullptr;
+  void destroy() {
+    if (prefix) (void)hipGraphExecDestroy(prefix);
+    if (passes) (void)hipGraphExecDestroy(passes);
+    if (suffix) (void)hipGraphExecDestroy(suffix);
+    prefix = passes = suffix = nullptr;
+  }
+};
 
 }  // namespace
 
 struct mptc_ctx {
   mptc_config cfg;
-  int alloc_S = 0;  // search area the E buffer is sized for
+  int alloc_S = 0;    // search area the E buffer is sized for
+  int max_total = 0;  // blocks per batch
+  int max_frames = 1;
+  size_t nbs = 0;     // E row stride (blocks)
   hipStream_t stream = nullptr;
-  int max_nb = 0;
-  size_t nbs = 0;  // E row stride (blocks)
-  // device buffers
-  uint8_t* d_rgb = nullptr;
-  uint64_t *d_own = nullptr, *d_ref = nullptr, *d_phys = nullptr;
-  uint64_t *d_inter = nullptr, *d_intra = nullptr;
-  uint32_t *d_rw = nullptr, *d_cw = nullptr;
+  mptc_tuning tune{256, 64, 2, 1, 2};
+  // device buffers (one entry per block of the batch unless noted)
+  FrameDev* d_fr = nullptr;  // kMaxFrames descriptors
+  FrameDev* h_fr = nullptr;  // pinned staging ring: 4 x kMaxFrames
+  int fr_slot = 0;
+  uint8_t *d_sid = nullptr, *d_tsid = nullptr, *d_need = nullptr;
+  uint64_t *d_own = nullptr, *d_inter = nullptr, *d_intra = nullptr;
+  uint32_t *d_rw = nullptr, *d_cw = nullptr, *d_cwn = nullptr, *d_dirty = nullptr, *d_tiles = nullptr;
   float4 *d_rtab = nullptr, *d_ctab = nullptr;
-  uint8_t *d_need = nullptr, *d_dirty = nullptr;
-  int16_t* d_E = nullptr;
-  int* d_chg = nullptr;
-  uint32_t* d_tiles = nullptr;
-  uint8_t *d_motion = nullptr, *d_eps = nullptr, *d_kind = nullptr;
-  uint32_t *d_unique = nullptr, *d_nunique = nullptr;
+  int *d_orig = nullptr, *d_chg = nullptr;
+  uint2* d_E = nullptr;  // S^2 groups of four int16 error deltas per block
+  uint4* d_summ = nullptr;  // kMaxChunks selection summaries per block
   DevStats* d_stats = nullptr;
   DevStats* h_stats = nullptr;  // pinned
-  int last_iters = 0, last_nb = 0, last_intra = 0;
+  uint32_t* h_pass = nullptr;   // pinned: changed-list length after each queued batch
+  std::vector<hipEvent_t> batch_ev;
+  // staging of the single-frame host-memory entry points
+  uint8_t* d_rgb = nullptr;
+  uint64_t *d_ref = nullptr, *d_phys = nullptr;
+  uint8_t *d_motion = nullptr, *d_eps = nullptr, *d_kind = nullptr;
+  uint32_t *d_unique = nullptr, *d_nunique = nullptr;
+  // batch shape the maps and graphs were built for
+  std::vector<int> shape;
+  mptc_tuning shape_tune{0, 0, 0, 0, 0};
+  int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0;
+  Graphs graphs;
+  int last_passes = 0, last_total = 0, last_intra = 0;
   // profiling
   bool prof = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -569,10 +191,12 @@ int launch(mptc_ctx* c, hipStream_t s, int kid, F&& f) {
 }
 
 int collect_profile(mptc_ctx* c) {
+  static const bool trace = std::getenv("MPTC_PROFILE_TRACE") != nullptr;
   for (auto& p : c->pending) {
     HIP_TRY(hipEventSynchronize(p.second.second));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    if (trace) std::fprintf(stderr, "mptc-trace %s %.4f\n", kKernelNames[p.first], ms);
     c->prof_n[p.first] += 1;
     c->prof_ms[p.first] += ms;
     c->spare.push_back(p.second);
@@ -587,6 +211,25 @@ int check_params(int S, int thr) {
   return MPTC_OK;
 }
 
+int check_tuning(const mptc_tuning& t) {
+  if (t.wg_size != 64 && t.wg_size != 128 && t.wg_size != 256)
+    return fail(MPTC_EINVAL, "wg_size must be 64, 128 or 256");
+  if (t.tile_w != 64 && t.tile_w != 32 && t.tile_w != 16 && t.tile_w != 8)
+    return fail(MPTC_EINVAL, "tile_w must be 64, 32, 16 or 8");
+  if (t.pass_batch < 1 || t.pass_batch > 16) return fail(MPTC_EINVAL, "pass_batch must be in 1..16");
+  if (t.win_chunks < 1 || t.win_chunks > kMaxChunks) return fail(MPTC_EINVAL, "win_chunks must be in 1..8");
+  return MPTC_OK;
+}
+
+// Window-row chunks actually used: the largest count up to the requested one
+// that divides the 2S window rows and keeps every chunk a whole number of
+// four-position E groups.
+int chunks_for(int S, int want) {
+  for (int k = std::min(want, kMaxChunks); k > 1; --k)
+    if ((2 * S) % k == 0 && ((2 * S / k) * 2 * S) % 4 == 0) return k;
+  return 1;
+}
+
 template <typename T>
 int dalloc(T** p, size_t n) {
   if (hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1)) != hipSuccess)
@@ -594,105 +237,262 @@ int dalloc(T** p, size_t n) {
   return MPTC_OK;
 }
 
-inline unsigned grid_for(int n) { return (unsigned)((n + kWG - 1) / kWG); }
+inline unsigned grid_for(int n, int wg) { return (unsigned)std::max(1, (n + wg - 1) / wg); }
 
-// One frame, all device pointers.  own_in: fitted blocks to start from
-// (nullptr -> fit on device from rgb).
-int run_frame(mptc_ctx* c, hipStream_t s, const uint8_t* rgb, int w, int h, int is_intra,
-              const uint64_t* ref, const uint64_t* own_in, uint64_t* phys_out, uint8_t* motion,
-              uint32_t* unique, uint32_t* n_unique, uint8_t* eps, uint8_t* kind) {
-  const int bw = w / 4, bh = h / 4, nb = bw * bh;
-  const int S = c->cfg.search_area, thr = c->cfg.err_threshold;
-  const size_t nbs = c->nbs;
-  const int n_tiles = (nb + kUniqueTile - 1) / kUniqueTile;
-  int rc;
-  // zero counters
-  if ((rc = launch(c, s, KF_ZERO, [&] {
-         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kWG), 0, s, reinterpret_cast<uint32_t*>(c->d_stats),
-                            (int)(sizeof(DevStats) / 4));
-       })))
-    return rc;
-  if ((rc = launch(c, s, KF_ZERO, [&] {
-         hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n_tiles)), dim3(kWG), 0, s, c->d_tiles, n_tiles);
-       })))
-    return rc;
-  const uint64_t* own = own_in;
-  if (!own) {
-    if ((rc = launch(c, s, KF_FIT, [&] {
-           hipLaunchKernelGGL(k_fit, dim3(grid_for(nb)), dim3(kWG), 0, s, rgb, w, bw, nb, c->d_own);
-         })))
-      return rc;
-    own = c->d_own;
+// One frame of a batch, host view.
+struct FrameIn {
+  const uint8_t* rgb;
+  int w, h, is_intra;
+  const uint64_t* ref;
+  uint64_t* phys;
+  uint8_t* motion;
+  uint32_t* unique;
+  uint32_t* n_unique;
+  uint8_t* eps;
+  uint8_t* kind;
+};
+
+// Lays out a batch (block, wave-tile and unique-tile offsets) and, when its
+// shape changed, rebuilds the block/tile frame maps and drops the graphs.
+int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
+  if (n < 1 || n > c->max_frames) return fail(MPTC_EINVAL, "batch size out of range");
+  const int tw = c->tune.tile_w, th = 64 / tw;
+  std::vector<int> shape;
+  FrameDev* h = c->h_fr + (size_t)c->fr_slot * kMaxFrames;
+  int base = 0, tb = 0, wt = 0;
+  for (int f = 0; f < n; ++f) {
+    const FrameIn& x = in[f];
+    if (x.w <= 0 || x.h <= 0 || (x.w & 3) || (x.h & 3))
+      return fail(MPTC_EINVAL, "width and height must be positive multiples of 4");
+    if (!x.rgb || !x.phys || !x.motion || !x.unique || !x.n_unique) return fail(MPTC_EINVAL, "null argument");
+    if (!x.is_intra && !x.ref) return fail(MPTC_EINVAL, "inter frame needs the previous frame's blocks");
+    FrameDev d{};
+    d.rgb = x.rgb; d.ref = x.ref; d.phys_out = x.phys; d.motion = x.motion; d.unique = x.unique;
+    d.n_unique = x.n_unique; d.eps = x.eps; d.kind = x.kind;
+    d.width = x.w; d.bw = x.w / 4; d.bh = x.h / 4; d.nb = d.bw * d.bh;
+    d.base = base; d.is_intra = x.is_intra ? 1 : 0;
+    d.tile_base = tb; d.n_tiles = (d.nb + kUniqueTile - 1) / kUniqueTile;
+    d.tiles_x = (d.bw + tw - 1) / tw;
+    d.wtile_base = wt;
+    base += d.nb;
+    tb += d.n_tiles;
+    wt += d.tiles_x * ((d.bh + th - 1) / th);
+    if (base > c->max_total) return fail(MPTC_EINVAL, "batch holds more blocks than the context allows");
+    h[f] = d;
+    shape.push_back(x.w);
+    shape.push_back(x.h);
   }
-  if (!is_intra) {
-    if ((rc = launch(c, s, KF_TABLE, [&] {
-           hipLaunchKernelGGL(k_table, dim3(grid_for(nb)), dim3(kWG), 0, s,
-                              reinterpret_cast<const uint32_t*>(ref) + 1, 2, nb, c->d_rw, c->d_rtab);
-         })))
-      return rc;
-    if ((rc = launch(c, s, KF_INTER, [&] {
-           hipLaunchKernelGGL(k_inter, dim3(grid_for(nb)), dim3(kWG), 0, s, rgb, w, bw, bh, S, thr, own,
-                              c->d_rw, c->d_rtab, c->d_inter, c->d_stats);
-         })))
-      return rc;
+  c->fr_slot = (c->fr_slot + 1) & 3;
+  HIP_TRY(hipMemcpyAsync(c->d_fr, h, sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
+  const bool same = shape == c->shape && std::memcmp(&c->tune, &c->shape_tune, sizeof(mptc_tuning)) == 0;
+  if (!same) {
+    // frame index of every wave tile
+    std::vector<uint8_t> tsid(std::max(wt, 1));
+    for (int f = 0; f < n; ++f) {
+      const int cnt = h[f].tiles_x * ((h[f].bh + (64 / tw) - 1) / (64 / tw));
+      std::fill(tsid.begin() + h[f].wtile_base, tsid.begin() + h[f].wtile_base + cnt, (uint8_t)f);
+    }
+    if (c->d_tsid) (void)hipFree(c->d_tsid);
+    c->d_tsid = nullptr;
+    int rc = dalloc(&c->d_tsid, tsid.size());
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->d_tsid, tsid.data(), tsid.size(), hipMemcpyHostToDevice, s));
+    int maxnb = 1;
+    for (int f = 0; f < n; ++f) maxnb = std::max(maxnb, h[f].nb);
+    hipLaunchKernelGGL(k_fill_sid, dim3(grid_for(maxnb, 256), n), dim3(256), 0, s, c->d_fr, n, c->d_sid);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));  // the tsid staging vector is released on return
+    c->graphs.destroy();
+    c->shape = shape;
+    c->shape_tune = c->tune;
   }
-  if ((rc = launch(c, s, KF_TABLE, [&] {
-         hipLaunchKernelGGL(k_intra_prep, dim3(grid_for(nb)), dim3(kWG), 0, s, nb, is_intra, own, c->d_inter,
-                            c->d_need, c->d_cw, c->d_ctab, c->d_dirty);
-       })))
-    return rc;
-  if ((rc = launch(c, s, KF_DENSE, [&] {
-         hipLaunchKernelGGL(k_intra_dense, dim3(grid_for(nb)), dim3(kWG), 0, s, rgb, w, bw, bh, S, thr, own,
-                            c->d_need, c->d_cw, c->d_ctab, c->d_E, nbs, c->d_intra, c->d_stats);
-       })))
-    return rc;
-  int iters = 0;
-  for (;;) {
-    // the system is raster-triangular: it reaches its fixpoint in at most nb + 1 passes
-    if (++iters > nb + 2) return fail(MPTC_ESTATE, "intra fixpoint did not converge");
-    if ((rc = launch(c, s, KF_UPDATE, [&] {
-           hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, &c->d_stats->n_changed, 1);
-           hipLaunchKernelGGL(k_intra_update, dim3(grid_for(nb)), dim3(kWG), 0, s, nb, own, c->d_need,
-                              c->d_intra, c->d_cw, c->d_ctab, c->d_chg, c->d_stats);
-         })))
-      return rc;
-    HIP_TRY(hipMemcpyAsync(c->h_stats, c->d_stats, sizeof(DevStats), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    const unsigned n_chg = c->h_stats->n_changed;
-    if (n_chg == 0) break;
-    const size_t work = (size_t)n_chg * (size_t)(4 * S * S);
-    const unsigned grid = (unsigned)std::min<size_t>((work + kWG - 1) / kWG, 8192);
-    if ((rc = launch(c, s, KF_REEVAL, [&] {
-           hipLaunchKernelGGL(k_intra_reeval, dim3(grid), dim3(kWG), 0, s, rgb, w, bw, bh, S, own, c->d_need,
-                              c->d_cw, c->d_ctab, c->d_chg, c->d_E, nbs, c->d_dirty, c->d_stats);
-         })))
-      return rc;
-    if ((rc = launch(c, s, KF_SELECT, [&] {
-           hipLaunchKernelGGL(k_intra_select, dim3(grid_for(nb)), dim3(kWG), 0, s, bw, bh, S, thr, c->d_cw,
-                              c->d_E, nbs, c->d_dirty, c->d_intra);
-         })))
-      return rc;
-  }
-  if ((rc = launch(c, s, KF_COMMIT, [&] {
-         hipLaunchKernelGGL(k_commit, dim3(grid_for(nb)), dim3(kWG), 0, s, rgb, w, bw, nb, is_intra, own,
-                            c->d_inter, c->d_need, c->d_intra, phys_out, motion, eps, kind, c->d_tiles,
-                            c->d_stats);
-       })))
-    return rc;
-  if ((rc = launch(c, s, KF_UNIQUE, [&] {
-         hipLaunchKernelGGL(k_unique, dim3(n_tiles), dim3(kWG), 0, s, nb, own, motion, c->d_tiles, n_tiles,
-                            unique, n_unique, c->d_stats);
-       })))
-    return rc;
-  c->last_iters = iters;
-  c->last_nb = nb;
-  c->last_intra = is_intra;
+  c->n_frames = n;
+  c->total = base;
+  c->n_wtiles = wt;
+  c->n_tiles = tb;
   return MPTC_OK;
 }
 
-int check_dims(mptc_ctx* c, int w, int h) {
-  if (w <= 0 || h <= 0 || (w & 3) || (h & 3)) return fail(MPTC_EINVAL, "width and height must be positive multiples of 4");
-  if ((w / 4) * (h / 4) > c->max_nb) return fail(MPTC_EINVAL, "frame larger than the context's maximum");
+// The three launch sequences of a step.  `fit` = false starts from
+// caller-provided fitted blocks already in d_own.
+int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
+  const int total = c->total, wg = c->tune.wg_size, tw = c->tune.tile_w;
+  const int S = c->cfg.search_area, thr = c->cfg.err_threshold;
+  const unsigned gw = grid_for(c->n_wtiles * 64, wg);
+  int rc;
+  if ((rc = launch(c, s, KF_ZERO, [&] {
+         hipLaunchKernelGGL(k_zero_u32, dim3(grid_for((int)(sizeof(DevStats) / 4), 256)), dim3(256), 0, s,
+                            reinterpret_cast<uint32_t*>(c->d_stats), (int)(sizeof(DevStats) / 4));
+         hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(c->n_tiles, 256)), dim3(256), 0, s, c->d_tiles, c->n_tiles);
+       })))
+    return rc;
+  if (fit && (rc = launch(c, s, KF_FIT, [&] {
+                hipLaunchKernelGGL(k_fit, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
+                                   c->d_own);
+              })))
+    return rc;
+  if ((rc = launch(c, s, KF_TABLE, [&] {
+         hipLaunchKernelGGL(k_table_ref, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
+                            c->d_rw, c->d_rtab);
+       })))
+    return rc;
+  const int K = chunks_for(S, c->tune.win_chunks);
+  const unsigned gwk = grid_for(c->n_wtiles * K * 64, wg);
+  if ((rc = launch(c, s, KF_INTER, [&] {
+         hipLaunchKernelGGL(k_inter, dim3(gwk), dim3(wg), 0, s, c->d_fr, c->d_tsid, c->n_wtiles, tw, K, S, c->d_own,
+                            c->d_rw, c->d_rtab, c->d_summ, c->nbs, c->d_stats);
+       })))
+    return rc;
+  if ((rc = launch(c, s, KF_INTER_COMBINE, [&] {
+         hipLaunchKernelGGL(k_inter_combine, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total, K,
+                            S, thr, c->d_rw, c->d_summ, c->nbs, c->d_inter);
+       })))
+    return rc;
+  if ((rc = launch(c, s, KF_PREP, [&] {
+         hipLaunchKernelGGL(k_intra_prep, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
+                            c->d_own, c->d_inter, c->d_need, c->d_cw, c->d_ctab, c->d_dirty, c->d_orig);
+       })))
+    return rc;
+  if ((rc = launch(c, s, KF_DENSE, [&] {
+         hipLaunchKernelGGL(k_intra_dense, dim3(gwk), dim3(wg), 0, s, c->d_fr, c->d_tsid, c->n_wtiles, tw, K, S,
+                            c->d_own, c->d_need, c->d_cw, c->d_ctab, c->d_orig, c->d_E, c->nbs, c->d_summ, c->nbs,
+                            c->d_stats);
+       })))
+    return rc;
+  if ((rc = launch(c, s, KF_INTRA_COMBINE, [&] {
+         hipLaunchKernelGGL(k_intra_combine, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total, K,
+                            S, thr, c->d_own, c->d_need, c->d_cw, c->d_summ, c->nbs, c->d_intra, c->d_cwn, c->d_chg,
+                            c->d_stats);
+       })))
+    return rc;
+  return launch(c, s, KF_APPLY, [&] {
+    hipLaunchKernelGGL(k_intra_apply, dim3(kListGrid), dim3(256), 0, s, c->d_cwn, c->d_cw, c->d_ctab, c->d_chg,
+                       c->d_stats);
+  });
+}
+
+// An even number of fixpoint passes, so that every batch starts and ends at
+// list parity 0.
+int passes_per_batch(const mptc_ctx* c) { return 2 * ((c->tune.pass_batch + 1) / 2); }
+
+int queue_passes(mptc_ctx* c, hipStream_t s) {
+  const int S = c->cfg.search_area, thr = c->cfg.err_threshold, total = c->total;
+  const int np = passes_per_batch(c);
+  int rc;
+  for (int p = 0; p < np; ++p) {
+    const int par = p & 1;
+    if ((rc = launch(c, s, KF_REEVAL, [&] {
+           hipLaunchKernelGGL(k_intra_reeval, dim3(kReevalGrid), dim3(256), 0, s, c->d_fr, c->d_sid, S, c->d_own,
+                              c->d_need, c->d_cw, c->d_ctab, c->d_chg, c->d_E, c->nbs, c->d_dirty, c->d_orig,
+                              c->d_intra, c->d_stats, par);
+         })))
+      return rc;
+    if ((rc = launch(c, s, KF_SELECT, [&] {
+           hipLaunchKernelGGL(k_intra_select, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total, S,
+                              thr, c->d_own, c->d_cw, c->d_ctab, c->d_E, c->nbs, c->d_dirty, c->d_intra, c->d_chg,
+                              c->d_stats, par ^ 1);
+         })))
+      return rc;
+  }
+  return MPTC_OK;
+}
+
+int queue_suffix(mptc_ctx* c, hipStream_t s) {
+  const int total = c->total, wg = c->tune.wg_size;
+  int rc;
+  if ((rc = launch(c, s, KF_COMMIT, [&] {
+         hipLaunchKernelGGL(k_commit, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total, c->d_own,
+                            c->d_inter, c->d_need, c->d_intra, c->d_tiles, c->d_stats);
+       })))
+    return rc;
+  return launch(c, s, KF_UNIQUE, [&] {
+    hipLaunchKernelGGL(k_unique, dim3(c->n_tiles), dim3(256), 0, s, c->d_fr, c->n_frames, c->d_own, c->d_tiles,
+                       c->d_stats);
+  });
+}
+
+template <typename Q>
+int capture(hipStream_t s, hipGraphExec_t* exec, Q&& q) {
+  HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = q();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(MPTC_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  const hipError_t e2 = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e2 != hipSuccess) return fail(MPTC_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+  return MPTC_OK;
+}
+
+// One encode step over the prepared batch.
+int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
+  const bool graphs = c->tune.use_graph && !c->prof && fit;
+  int rc;
+  if (graphs && !c->graphs.prefix) {
+    if ((rc = capture(s, &c->graphs.prefix, [&] { return queue_prefix(c, s, true); }))) return rc;
+    if ((rc = capture(s, &c->graphs.passes, [&] { return queue_passes(c, s); }))) return rc;
+    if ((rc = capture(s, &c->graphs.suffix, [&] { return queue_suffix(c, s); }))) return rc;
+  }
+  if (graphs) {
+    HIP_TRY(hipGraphLaunch(c->graphs.prefix, s));
+  } else if ((rc = queue_prefix(c, s, fit))) {
+    return rc;
+  }
+  if (c->batch_ev.empty()) {
+    c->batch_ev.resize(2);
+    for (auto& e : c->batch_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  const int np = passes_per_batch(c);
+  int passes = 0;
+  auto queue_batch = [&](int slot) -> int {
+    if (graphs) {
+      HIP_TRY(hipGraphLaunch(c->graphs.passes, s));
+    } else {
+      const int r = queue_passes(c, s);
+      if (r) return r;
+    }
+    passes += np;
+    // length of the changed list the batch's last pass produced (parity 0)
+    HIP_TRY(hipMemcpyAsync(&c->h_pass[slot], &c->d_stats->n_changed[0], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipEventRecord(c->batch_ev[slot], s));
+    return MPTC_OK;
+  };
+  int slot = 0;
+  if ((rc = queue_batch(slot))) return rc;
+  for (;;) {
+    // the raster-triangular system settles within one pass per block
+    if (passes > c->total + 4 * np) return fail(MPTC_ESTATE, "intra fixpoint did not converge");
+    if ((rc = queue_batch(slot ^ 1))) return rc;
+    HIP_TRY(hipEventSynchronize(c->batch_ev[slot]));
+    if (c->h_pass[slot] == 0) break;  // converged; the batch queued after it runs empty
+    slot ^= 1;
+  }
+  if (graphs) {
+    HIP_TRY(hipGraphLaunch(c->graphs.suffix, s));
+  } else if ((rc = queue_suffix(c, s))) {
+    return rc;
+  }
+  c->last_passes = passes;
+  c->last_total = c->total;
+  return MPTC_OK;
+}
+
+int encode_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in, bool fit) {
+  int rc = prepare_batch(c, s, n, in);
+  if (rc) return rc;
+  c->last_intra = n == 1 ? in[0].is_intra : 0;
+  return run_step(c, s, fit);
+}
+
+int one_frame_dims(mptc_ctx* c, int w, int h) {
+  if (w <= 0 || h <= 0 || (w & 3) || (h & 3))
+    return fail(MPTC_EINVAL, "width and height must be positive multiples of 4");
+  if ((long long)(w / 4) * (h / 4) > (long long)(c->cfg.max_width / 4) * (c->cfg.max_height / 4))
+    return fail(MPTC_EINVAL, "frame larger than the context's maximum");
   return MPTC_OK;
 }
 
@@ -709,6 +509,11 @@ int32_t mptc_create(const mptc_config* cfg, mptc_ctx** out) {
     return fail(MPTC_EINVAL, "max_width/max_height must be positive multiples of 4");
   int rc = check_params(cfg->search_area, cfg->err_threshold);
   if (rc) return rc;
+  const int frames = cfg->max_batch_frames <= 0 ? 1 : cfg->max_batch_frames;
+  if (frames > kMaxFrames) return fail(MPTC_EINVAL, "max_batch_frames above 64");
+  const long long one = (long long)(cfg->max_width / 4) * (cfg->max_height / 4);
+  const long long total = std::max(one, cfg->max_batch_blocks > 0 ? (long long)cfg->max_batch_blocks : one * frames);
+  if (total > (1ll << 30)) return fail(MPTC_EINVAL, "batch too large");
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));
   if (cfg->device < 0 || cfg->device >= ndev) return fail(MPTC_EINVAL, "no such HIP device");
@@ -717,8 +522,9 @@ int32_t mptc_create(const mptc_config* cfg, mptc_ctx** out) {
   mptc_ctx* c = new mptc_ctx();
   c->cfg = *cfg;
   c->alloc_S = cfg->search_area;
-  const int nb = (cfg->max_width / 4) * (cfg->max_height / 4);
-  c->max_nb = nb;
+  c->max_frames = frames;
+  c->max_total = (int)total;
+  const int nb = c->max_total;
   c->nbs = (size_t)((nb + 63) / 64) * 64;
   auto cleanup = [&](int code) {
     mptc_destroy(c);
@@ -726,18 +532,23 @@ int32_t mptc_create(const mptc_config* cfg, mptc_ctx** out) {
   };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(MPTC_EHIP, "hipStreamCreate failed"));
-  const int n_tiles = (nb + kUniqueTile - 1) / kUniqueTile;
-  const size_t e_slots = 4 * (size_t)cfg->search_area * cfg->search_area;
-  if ((rc = dalloc(&c->d_rgb, (size_t)nb * 48)) || (rc = dalloc(&c->d_own, nb)) || (rc = dalloc(&c->d_ref, nb)) ||
-      (rc = dalloc(&c->d_phys, nb)) || (rc = dalloc(&c->d_inter, nb)) || (rc = dalloc(&c->d_intra, nb)) ||
-      (rc = dalloc(&c->d_rw, nb)) || (rc = dalloc(&c->d_cw, nb)) || (rc = dalloc(&c->d_rtab, nb)) ||
-      (rc = dalloc(&c->d_ctab, nb)) || (rc = dalloc(&c->d_need, nb)) || (rc = dalloc(&c->d_dirty, nb)) ||
-      (rc = dalloc(&c->d_E, e_slots * c->nbs)) || (rc = dalloc(&c->d_chg, nb)) ||
-      (rc = dalloc(&c->d_tiles, n_tiles)) || (rc = dalloc(&c->d_motion, (size_t)nb * 2)) ||
-      (rc = dalloc(&c->d_eps, (size_t)nb * 6)) || (rc = dalloc(&c->d_kind, nb)) ||
-      (rc = dalloc(&c->d_unique, nb)) || (rc = dalloc(&c->d_nunique, 1)) || (rc = dalloc(&c->d_stats, 1)))
+  const int max_tiles = nb / kUniqueTile + frames + 1;
+  const size_t groups = (size_t)cfg->search_area * cfg->search_area;
+  const size_t one_nb = (size_t)one;
+  if ((rc = dalloc(&c->d_fr, kMaxFrames)) || (rc = dalloc(&c->d_sid, nb)) || (rc = dalloc(&c->d_need, nb)) ||
+      (rc = dalloc(&c->d_own, nb)) || (rc = dalloc(&c->d_inter, nb)) || (rc = dalloc(&c->d_intra, nb)) ||
+      (rc = dalloc(&c->d_rw, nb)) || (rc = dalloc(&c->d_cw, nb)) || (rc = dalloc(&c->d_cwn, nb)) ||
+      (rc = dalloc(&c->d_dirty, nb)) || (rc = dalloc(&c->d_tiles, max_tiles)) || (rc = dalloc(&c->d_rtab, nb)) ||
+      (rc = dalloc(&c->d_ctab, nb)) || (rc = dalloc(&c->d_orig, nb)) || (rc = dalloc(&c->d_chg, nb)) ||
+      (rc = dalloc(&c->d_E, groups * c->nbs)) || (rc = dalloc(&c->d_summ, kMaxChunks * c->nbs)) || (rc = dalloc(&c->d_stats, 1)) ||
+      (rc = dalloc(&c->d_rgb, one_nb * 48)) || (rc = dalloc(&c->d_ref, one_nb)) || (rc = dalloc(&c->d_phys, one_nb)) ||
+      (rc = dalloc(&c->d_motion, one_nb * 2)) || (rc = dalloc(&c->d_eps, one_nb * 6)) ||
+      (rc = dalloc(&c->d_kind, one_nb)) || (rc = dalloc(&c->d_unique, one_nb)) || (rc = dalloc(&c->d_nunique, 1)))
     return cleanup(rc);
-  if (hipHostMalloc(reinterpret_cast<void**>(&c->h_stats), sizeof(DevStats), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->h_stats), sizeof(DevStats), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->h_pass), 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->h_fr), 4 * kMaxFrames * sizeof(FrameDev), hipHostMallocDefault) !=
+          hipSuccess)
     return cleanup(fail(MPTC_ENOMEM, "hipHostMalloc failed"));
   std::memset(c->h_stats, 0, sizeof(DevStats));
   *out = c;
@@ -748,14 +559,19 @@ void mptc_destroy(mptc_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->cfg.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->graphs.destroy();
   for (auto& p : c->pending) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
   for (auto& p : c->spare) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
-  void* bufs[] = {c->d_rgb, c->d_own, c->d_ref, c->d_phys, c->d_inter, c->d_intra, c->d_rw, c->d_cw,
-                  c->d_rtab, c->d_ctab, c->d_need, c->d_dirty, c->d_E, c->d_chg, c->d_tiles, c->d_motion,
-                  c->d_eps, c->d_kind, c->d_unique, c->d_nunique, c->d_stats};
+  for (hipEvent_t e : c->batch_ev) (void)hipEventDestroy(e);
+  void* bufs[] = {c->d_fr,   c->d_sid,   c->d_tsid,   c->d_need,  c->d_own,   c->d_inter,  c->d_intra,
+                  c->d_rw,   c->d_cw,    c->d_cwn,    c->d_dirty, c->d_tiles, c->d_rtab,   c->d_ctab,
+                  c->d_orig, c->d_chg,   c->d_E, c->d_summ,     c->d_stats, c->d_rgb,    c->d_ref,
+                  c->d_phys, c->d_motion, c->d_eps,   c->d_kind,  c->d_unique, c->d_nunique};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_stats) (void)hipHostFree(c->h_stats);
+  if (c->h_pass) (void)hipHostFree(c->h_pass);
+  if (c->h_fr) (void)hipHostFree(c->h_fr);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -765,21 +581,38 @@ int32_t mptc_set_params(mptc_ctx* c, int32_t S, int32_t thr) {
   int rc = check_params(S, thr);
   if (rc) return rc;
   if (S > c->alloc_S) return fail(MPTC_EINVAL, "search_area above the one the context was created with");
+  if (S != c->cfg.search_area || thr != c->cfg.err_threshold) c->graphs.destroy();
   c->cfg.search_area = S;
   c->cfg.err_threshold = thr;
   return MPTC_OK;
 }
 
+int32_t mptc_get_tuning(mptc_ctx* c, mptc_tuning* out) {
+  if (!c || !out) return fail(MPTC_EINVAL, "null argument");
+  *out = c->tune;
+  return MPTC_OK;
+}
+
+int32_t mptc_set_tuning(mptc_ctx* c, const mptc_tuning* t) {
+  if (!c || !t) return fail(MPTC_EINVAL, "null argument");
+  int rc = check_tuning(*t);
+  if (rc) return rc;
+  c->tune = *t;
+  return MPTC_OK;
+}
+
 int32_t mptc_fit(mptc_ctx* c, const uint8_t* rgb, int32_t w, int32_t h, uint64_t* phys_out) {
   if (!c || !rgb || !phys_out) return fail(MPTC_EINVAL, "null argument");
-  int rc = check_dims(c, w, h);
+  int rc = one_frame_dims(c, w, h);
   if (rc) return rc;
   HIP_TRY(hipSetDevice(c->cfg.device));
-  const int bw = w / 4, nb = bw * (h / 4);
   hipStream_t s = c->stream;
+  const int nb = (w / 4) * (h / 4);
   HIP_TRY(hipMemcpyAsync(c->d_rgb, rgb, (size_t)w * h * 3, hipMemcpyHostToDevice, s));
+  FrameIn in{c->d_rgb, w, h, 1, nullptr, c->d_phys, c->d_motion, c->d_unique, c->d_nunique, nullptr, nullptr};
+  if ((rc = prepare_batch(c, s, 1, &in))) return rc;
   if ((rc = launch(c, s, KF_FIT, [&] {
-         hipLaunchKernelGGL(k_fit, dim3(grid_for(nb)), dim3(kWG), 0, s, c->d_rgb, w, bw, nb, c->d_own);
+         hipLaunchKernelGGL(k_fit, dim3(grid_for(nb, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, nb, c->d_own);
        })))
     return rc;
   HIP_TRY(hipMemcpyAsync(phys_out, c->d_own, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
@@ -788,12 +621,12 @@ int32_t mptc_fit(mptc_ctx* c, const uint8_t* rgb, int32_t w, int32_t h, uint64_t
 }
 
 int32_t mptc_reencode(mptc_ctx* c, const uint8_t* rgb, int32_t w, int32_t h, int32_t is_intra,
-                      const uint64_t* ref_phys, uint64_t* phys_inout, uint8_t* motion_out,
-                      uint32_t* unique_out, uint32_t* n_unique_out, uint8_t* eps_out, uint8_t* kind_out) {
+                      const uint64_t* ref_phys, uint64_t* phys_inout, uint8_t* motion_out, uint32_t* unique_out,
+                      uint32_t* n_unique_out, uint8_t* eps_out, uint8_t* kind_out) {
   if (!c || !rgb || !phys_inout || !motion_out || !unique_out || !n_unique_out)
     return fail(MPTC_EINVAL, "null argument");
   if (!is_intra && !ref_phys) return fail(MPTC_EINVAL, "inter frame needs ref_phys");
-  int rc = check_dims(c, w, h);
+  int rc = one_frame_dims(c, w, h);
   if (rc) return rc;
   HIP_TRY(hipSetDevice(c->cfg.device));
   const int nb = (w / 4) * (h / 4);
@@ -801,9 +634,9 @@ int32_t mptc_reencode(mptc_ctx* c, const uint8_t* rgb, int32_t w, int32_t h, int
   HIP_TRY(hipMemcpyAsync(c->d_rgb, rgb, (size_t)w * h * 3, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->d_own, phys_inout, (size_t)nb * 8, hipMemcpyHostToDevice, s));
   if (!is_intra) HIP_TRY(hipMemcpyAsync(c->d_ref, ref_phys, (size_t)nb * 8, hipMemcpyHostToDevice, s));
-  if ((rc = run_frame(c, s, c->d_rgb, w, h, is_intra, c->d_ref, c->d_own, c->d_phys, c->d_motion, c->d_unique,
-                      c->d_nunique, eps_out ? c->d_eps : nullptr, kind_out ? c->d_kind : nullptr)))
-    return rc;
+  FrameIn in{c->d_rgb,    w, h, is_intra, is_intra ? nullptr : c->d_ref, c->d_phys, c->d_motion, c->d_unique,
+             c->d_nunique, eps_out ? c->d_eps : nullptr, kind_out ? c->d_kind : nullptr};
+  if ((rc = encode_batch(c, s, 1, &in, false))) return rc;
   uint32_t n_unique = 0;
   HIP_TRY(hipMemcpyAsync(&n_unique, c->d_nunique, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(phys_inout, c->d_phys, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
@@ -816,19 +649,77 @@ int32_t mptc_reencode(mptc_ctx* c, const uint8_t* rgb, int32_t w, int32_t h, int
   return MPTC_OK;
 }
 
+int32_t mptc_encode_batch_device(mptc_ctx* c, int32_t n, const mptc_frame_io* frames, void* hip_stream) {
+  if (!c || !frames) return fail(MPTC_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(c->cfg.device));
+  if (n < 1 || n > c->max_frames) return fail(MPTC_EINVAL, "batch size out of range");
+  FrameIn in[kMaxFrames];
+  for (int f = 0; f < n; ++f) {
+    const mptc_frame_io& x = frames[f];
+    in[f] = FrameIn{x.d_rgb,        x.width,        x.height,       x.is_intra, x.d_ref_phys, x.d_phys_out,
+                    x.d_motion_out, x.d_unique_out, x.d_n_unique_out, x.d_eps_out, x.d_kind_out};
+  }
+  hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : c->stream;
+  return encode_batch(c, s, n, in, true);
+}
+
 int32_t mptc_encode_frame_device(mptc_ctx* c, const uint8_t* d_rgb, int32_t w, int32_t h, int32_t is_intra,
                                  const uint64_t* d_ref_phys, uint64_t* d_phys_out, uint8_t* d_motion_out,
                                  uint32_t* d_unique_out, uint32_t* d_n_unique_out, uint8_t* d_eps_out,
                                  uint8_t* d_kind_out, void* hip_stream) {
-  if (!c || !d_rgb || !d_phys_out || !d_motion_out || !d_unique_out || !d_n_unique_out)
-    return fail(MPTC_EINVAL, "null argument");
-  if (!is_intra && !d_ref_phys) return fail(MPTC_EINVAL, "inter frame needs d_ref_phys");
-  int rc = check_dims(c, w, h);
+  if (!c) return fail(MPTC_EINVAL, "null context");
+  int rc = one_frame_dims(c, w, h);
   if (rc) return rc;
+  const mptc_frame_io io{d_rgb,          w, h, is_intra, d_ref_phys, d_phys_out, d_motion_out, d_unique_out,
+                         d_n_unique_out, d_eps_out, d_kind_out};
+  return mptc_encode_batch_device(c, 1, &io, hip_stream);
+}
+
+int32_t mptc_autotune(mptc_ctx* c, int32_t n, const mptc_frame_io* frames, int32_t steps, mptc_tuning* best,
+                      double* best_ms) {
+  if (!c || !frames || steps < 1) return fail(MPTC_EINVAL, "bad argument");
   HIP_TRY(hipSetDevice(c->cfg.device));
-  hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : c->stream;
-  return run_frame(c, s, d_rgb, w, h, is_intra, d_ref_phys, nullptr, d_phys_out, d_motion_out, d_unique_out,
-                   d_n_unique_out, d_eps_out, d_kind_out);
+  const mptc_tuning keep = c->tune;
+  mptc_tuning win = keep;
+  double win_ms = 1e30;
+  const int wgs[] = {64, 128, 256};
+  const int tws[] = {64, 32, 16, 8};
+  const int chunks[] = {1, 2, 4, 8};
+  for (int wg : wgs)
+    for (int tw : tws)
+      for (int kc : chunks) {
+        c->tune = mptc_tuning{wg, tw, keep.pass_batch, keep.use_graph, kc};
+        int rc = mptc_encode_batch_device(c, n, frames, nullptr);  // warm-up: maps and graphs
+        if (rc) {
+          c->tune = keep;
+          return rc;
+        }
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < steps; ++i)
+          if ((rc = mptc_encode_batch_device(c, n, frames, nullptr))) {
+            c->tune = keep;
+            return rc;
+          }
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+        if (ms < win_ms) {
+          win_ms = ms;
+          win = c->tune;
+        }
+      }
+  c->tune = win;
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
+  return MPTC_OK;
+}
+
+int32_t mptc_synchronize(mptc_ctx* c) {
+  if (!c) return fail(MPTC_EINVAL, "null context");
+  HIP_TRY(hipSetDevice(c->cfg.device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MPTC_OK;
 }
 
 int32_t mptc_get_stats(mptc_ctx* c, mptc_frame_stats* out) {
@@ -837,14 +728,17 @@ int32_t mptc_get_stats(mptc_ctx* c, mptc_frame_stats* out) {
   HIP_TRY(hipDeviceSynchronize());
   DevStats d;
   HIP_TRY(hipMemcpy(&d, c->d_stats, sizeof(d), hipMemcpyDeviceToHost));
-  out->n_blocks = c->last_nb;
-  out->n_inter = d.n_inter;
-  out->n_intra = d.n_intra;
+  out->n_blocks = c->last_total;
+  out->n_inter = out->n_intra = out->inter_candidates = out->intra_candidates = out->reeval_candidates = 0;
+  for (int i = 0; i < kStatSlots; ++i) {
+    out->n_inter += d.n_inter[i];
+    out->n_intra += d.n_intra[i];
+    out->inter_candidates += (int64_t)d.inter_cands[i];
+    out->intra_candidates += (int64_t)d.intra_cands[i];
+    out->reeval_candidates += (int64_t)d.reeval_cands[i];
+  }
   out->n_unique = d.n_unique;
-  out->inter_candidates = (int64_t)d.inter_cands;
-  out->intra_candidates = (int64_t)d.intra_cands;
-  out->reeval_candidates = (int64_t)d.reeval_cands;
-  out->intra_iterations = c->last_iters;
+  out->intra_iterations = (int32_t)d.active_passes + 1;
   out->is_intra = c->last_intra;
   return MPTC_OK;
 }

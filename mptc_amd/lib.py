@@ -26,7 +26,23 @@ class MptcError(RuntimeError):
 
 class Config(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_width", C.c_int32), ("max_height", C.c_int32),
-                ("search_area", C.c_int32), ("err_threshold", C.c_int32)]
+                ("search_area", C.c_int32), ("err_threshold", C.c_int32),
+                ("max_batch_frames", C.c_int32), ("max_batch_blocks", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("wg_size", C.c_int32), ("tile_w", C.c_int32), ("pass_batch", C.c_int32),
+                ("use_graph", C.c_int32), ("win_chunks", C.c_int32)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class FrameIO(C.Structure):
+    _fields_ = [("d_rgb", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32),
+                ("is_intra", C.c_int32), ("d_ref_phys", C.c_void_p), ("d_phys_out", C.c_void_p),
+                ("d_motion_out", C.c_void_p), ("d_unique_out", C.c_void_p),
+                ("d_n_unique_out", C.c_void_p), ("d_eps_out", C.c_void_p), ("d_kind_out", C.c_void_p)]
 
 
 class FrameStats(C.Structure):
@@ -71,9 +87,15 @@ def load() -> C.CDLL:
     L.mptc_encode_frame_device.argtypes = [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp,
                                            _vp, _vp, _vp, _vp, _vp, _vp]
     L.mptc_get_stats.argtypes = [_vp, C.POINTER(FrameStats)]
+    L.mptc_synchronize.argtypes = [_vp]
     L.mptc_profile_enable.argtypes = [_vp, C.c_int32]
     L.mptc_profile_read.argtypes = [_vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.mptc_synth_frames.argtypes = [_vp, C.c_int32, C.c_int32, _vp]
+    L.mptc_encode_batch_device.argtypes = [_vp, C.c_int32, C.POINTER(FrameIO), _vp]
+    L.mptc_get_tuning.argtypes = [_vp, C.POINTER(Tuning)]
+    L.mptc_set_tuning.argtypes = [_vp, C.POINTER(Tuning)]
+    L.mptc_autotune.argtypes = [_vp, C.c_int32, C.POINTER(FrameIO), C.c_int32, C.POINTER(Tuning),
+                                C.POINTER(C.c_double)]
     for name in header_functions():
         if name not in ("mptc_destroy", "mptc_last_error"):
             getattr(L, name).restype = C.c_int32
@@ -105,10 +127,12 @@ class Encoder:
     """One context: device work buffers for frames up to max_width x max_height."""
 
     def __init__(self, max_width: int, max_height: int, search_area: int = 8,
-                 err_threshold: int = 0, device: int = 0):
+                 err_threshold: int = 0, device: int = 0, max_batch_frames: int = 1,
+                 max_batch_blocks: int = 0):
         L = load()
         self.L = L
-        self.cfg = Config(device, max_width, max_height, search_area, err_threshold)
+        self.cfg = Config(device, max_width, max_height, search_area, err_threshold,
+                          max_batch_frames, max_batch_blocks)
         h = C.c_void_p()
         _check(L.mptc_create(C.byref(self.cfg), C.byref(h)))
         self.h = h
@@ -162,6 +186,49 @@ class Encoder:
                                                _ptr(d_ref), _ptr(d_phys), _ptr(d_motion),
                                                _ptr(d_unique), _ptr(d_n_unique), _ptr(d_eps),
                                                _ptr(d_kind), stream))
+        if stream is None:
+            self.synchronize()
+
+    def encode_batch_device(self, frames: list[dict], stream: int | None = None):
+        """frames: dicts with d_rgb, width, height, is_intra, d_ref, d_phys, d_motion,
+        d_unique, d_n_unique and optional d_eps, d_kind (device buffers)."""
+        arr = (FrameIO * len(frames))()
+        for i, f in enumerate(frames):
+            arr[i] = FrameIO(_ptr(f["d_rgb"]), f["width"], f["height"], int(bool(f["is_intra"])),
+                             _ptr(f.get("d_ref")), _ptr(f["d_phys"]), _ptr(f["d_motion"]),
+                             _ptr(f["d_unique"]), _ptr(f["d_n_unique"]), _ptr(f.get("d_eps")),
+                             _ptr(f.get("d_kind")))
+        _check(self.L.mptc_encode_batch_device(self.h, len(frames), arr, stream))
+        if stream is None:
+            self.synchronize()
+
+    def frame_io_array(self, frames: list[dict]):
+        arr = (FrameIO * len(frames))()
+        for i, f in enumerate(frames):
+            arr[i] = FrameIO(_ptr(f["d_rgb"]), f["width"], f["height"], int(bool(f["is_intra"])),
+                             _ptr(f.get("d_ref")), _ptr(f["d_phys"]), _ptr(f["d_motion"]),
+                             _ptr(f["d_unique"]), _ptr(f["d_n_unique"]), _ptr(f.get("d_eps")),
+                             _ptr(f.get("d_kind")))
+        return arr
+
+    def synchronize(self):
+        """Waits for the work queued on the context's own stream."""
+        _check(self.L.mptc_synchronize(self.h))
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _check(self.L.mptc_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, wg_size=256, tile_w=64, pass_batch=2, use_graph=1, win_chunks=2):
+        t = Tuning(wg_size, tile_w, pass_batch, use_graph, win_chunks)
+        _check(self.L.mptc_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, frames: list[dict], steps: int = 3):
+        arr = self.frame_io_array(frames)
+        best, ms = Tuning(), C.c_double()
+        _check(self.L.mptc_autotune(self.h, len(frames), arr, steps, C.byref(best), C.byref(ms)))
+        return best.as_dict(), ms.value
 
     def stats(self) -> dict:
         s = FrameStats()

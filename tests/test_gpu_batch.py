@@ -1,0 +1,121 @@
+"""Batched multi-stream steps, launch-shape tuning and graph replay: every
+configuration must give the same bytes as the single-stream path (which the
+parity tests pin to the reference)."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from mptc_amd.workloads import BATCH_STREAMS, SEQUENCES, synth_cfg
+from tests.helpers import compare_frame, compare_results, load_seq
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+class Stream:
+    """Device buffers of one stream: all its frames, two output slots."""
+
+    def __init__(self, hip, frames: np.ndarray):
+        n, h, w, _ = frames.shape
+        self.w, self.h, self.n = w, h, n
+        self.nb = (w // 4) * (h // 4)
+        self.rgb = hip.Buffer.from_array(frames)
+        self.phys = [hip.Buffer(8 * self.nb) for _ in range(2)]
+        self.motion = hip.Buffer(2 * self.nb)
+        self.unique = hip.Buffer(4 * self.nb)
+        self.n_unique = hip.Buffer(4)
+        self.eps = hip.Buffer(6 * self.nb)
+        self.kind = hip.Buffer(self.nb)
+
+    def io(self, f: int, is_intra: bool):
+        return dict(d_rgb=self.rgb.offset(f * self.w * self.h * 3), width=self.w, height=self.h,
+                    is_intra=is_intra, d_ref=None if is_intra else self.phys[(f + 1) % 2],
+                    d_phys=self.phys[f % 2], d_motion=self.motion, d_unique=self.unique,
+                    d_n_unique=self.n_unique, d_eps=self.eps, d_kind=self.kind)
+
+    def result(self, f: int):
+        n = int(self.n_unique.download(np.uint32, 1)[0])
+        return dict(phys=self.phys[f % 2].download(np.uint64, self.nb),
+                    motion=self.motion.download(np.uint8, 2 * self.nb),
+                    unique=self.unique.download(np.uint32, n), eps=self.eps.download(np.uint8, 6 * self.nb),
+                    kind=self.kind.download(np.uint8, self.nb))
+
+
+def _encode_streams(enc, streams, n_frames, intra_interval):
+    """Encodes n_frames steps of every stream as batches; returns per-step results."""
+    from mptc_amd.lib import next_intra
+    out = []
+    set_intra, num = True, 1
+    for f in range(n_frames):
+        enc.encode_batch_device([s.io(f, set_intra) for s in streams])
+        out.append([s.result(f) for s in streams])
+        set_intra, num = next_intra(set_intra, num, intra_interval)
+    return out
+
+
+def test_batch_of_uneven_streams_matches_reference(orc):
+    from mptc_amd import hip, lib
+    names = BATCH_STREAMS
+    specs = [SEQUENCES[n] for n in names]
+    n_frames = min(s["golden_frames"] for s in specs)
+    fxs = [load_seq(n) for n in names]
+    if any(fx is None for fx in fxs):
+        pytest.skip("stream fixtures not generated")
+    total = sum((s["width"] // 4) * (s["height"] // 4) for s in specs)
+    enc = lib.Encoder(3840, 2160, specs[0]["S"], specs[0]["thr"], max_batch_frames=len(specs),
+                      max_batch_blocks=total)
+    streams = [Stream(hip, orc.synth_frames(synth_cfg(s), n_frames)) for s in specs]
+    log(f"batch: {len(streams)} streams, {total} blocks per step, {n_frames} steps")
+    res = _encode_streams(enc, streams, n_frames, specs[0]["intra_interval"])
+    for f in range(n_frames):
+        for i, name in enumerate(names):
+            compare_frame(res[f][i], fxs[i]["frames"][f], f"{name} frame {f}")
+    enc.close()
+
+
+@pytest.mark.parametrize("wg,tw,pb,graph,chunks", [(256, 64, 2, 1, 2), (64, 8, 1, 0, 1), (128, 16, 2, 1, 4),
+                                                   (256, 32, 3, 0, 8), (64, 64, 16, 1, 6), (256, 16, 1, 1, 3)])
+def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks):
+    from mptc_amd import hip, lib
+    from oracle import SynthCfg
+    sizes = [(160, 96, 11), (96, 64, 12), (256, 128, 13)]
+    frames = [orc.synth_frames(SynthCfg(w, h, sd, 6, 1, 4, 8, 100), 4) for w, h, sd in sizes]
+    S, thr = 6, 2
+    want = [orc.encode_sequence(fr, S, thr, 3) for fr in frames]
+    total = sum((w // 4) * (h // 4) for w, h, _ in sizes)
+    enc = lib.Encoder(256, 128, S, thr, max_batch_frames=3, max_batch_blocks=total)
+    enc.set_tuning(wg, tw, pb, graph, chunks)
+    streams = [Stream(hip, fr) for fr in frames]
+    log(f"tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks}")
+    res = _encode_streams(enc, streams, 4, 3)
+    for f in range(4):
+        for i in range(3):
+            compare_results(res[f][i], want[i][f], f"stream {i} frame {f}")
+            assert np.array_equal(res[f][i]["kind"], want[i][f]["kind"])
+    enc.close()
+
+
+def test_autotune_picks_a_valid_shape(orc):
+    from mptc_amd import hip, lib
+    spec = SEQUENCES["c1_s8_t8"]
+    frames = orc.synth_frames(synth_cfg(spec), 2)
+    enc = lib.Encoder(256, 256, spec["S"], spec["thr"])
+    st = Stream(hip, frames)
+    enc.encode_batch_device([st.io(0, True)])
+    best, ms = enc.autotune([st.io(1, False)], steps=2)
+    log(f"autotune best {best} {ms:.3f} ms/step")
+    assert best["wg_size"] in (64, 128, 256) and best["tile_w"] in (64, 32, 16, 8) and ms > 0
+    assert enc.get_tuning() == best
+    # the tuned shape still encodes bit-exactly
+    enc.encode_batch_device([st.io(0, True)])
+    enc.encode_batch_device([st.io(1, False)])
+    want = orc.encode_sequence(frames, spec["S"], spec["thr"], spec["intra_interval"])
+    compare_results(st.result(1), want[1], "autotuned frame 1")
+    enc.close()
