@@ -156,10 +156,91 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
  * mptc_profile_enable(ctx, 1) starts recording; mptc_profile_read returns the
  * accumulated launch count and milliseconds of kernel `name` ("fit", "table",
  * "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine",
- * "intra_apply", "intra_reeval", "intra_select", "commit", "unique", "zero") since it was enabled
+ * "intra_apply", "intra_reeval", "intra_select", "commit", "unique", "zero";
+ * learned search: "net_input", "net_forward", "propose_inter", "top_intra",
+ * "lintra_pass") since it was enabled
  * (synchronises).  While enabled, steps run as individual launches, not graphs. */
 int32_t mptc_profile_enable(mptc_ctx* ctx, int32_t enable);
 int32_t mptc_profile_read(mptc_ctx* ctx, const char* name, int64_t* launches, double* total_ms);
+
+/* ---------------------------------------------------------------------------
+ * Learned proposals (no counterpart in the reference).  Two networks of one
+ * shape, MPTC_NET_INTER for the inter window and MPTC_NET_INTRA for the causal
+ * intra window: 48 inputs (a block's pixels, pixel-major RGB, / 255) -> 256
+ * ReLU -> 256 ReLU -> 262 outputs (256 logits over the 16x16 offsets of the
+ * window, then 6 endpoint seeds: ep1 rgb, ep2 rgb / 255).  BF16 weights and
+ * activations, FP32 accumulation, FP32 biases and master weights.
+ *
+ * Logit index of a window offset (dx = i - bx, in -8..7):
+ *   inter: (dy + 8) * 16 + dx + 8 with dy = j - by in -8..7
+ *   intra: dy * 16 + dx + 8        with dy = by - j in 0..15 (dy = 0: dx < 0 only)
+ *
+ * Parameters are one fp32 vector of MPTC_NET_PARAMS values: W1 [48][256],
+ * b1 [256], W2 [256][256], b2 [256], W3 [256][262], b3 [262] (W[k][n]: input
+ * k, output n).  Training loss: softmax cross-entropy of the logits against
+ * the chosen offset + seed_weight * mean squared error of the seeds against
+ * the re-fit endpoints / 255, averaged over the batch; Adam.
+ * ------------------------------------------------------------------------- */
+#define MPTC_NET_INTER 0
+#define MPTC_NET_INTRA 1
+#define MPTC_NET_IN 48
+#define MPTC_NET_HIDDEN 256
+#define MPTC_NET_OUT 262
+#define MPTC_NET_LOGITS 256
+#define MPTC_NET_PARAMS 145670
+
+/* Search modes (mptc_set_search_mode). */
+#define MPTC_SEARCH_EXHAUSTIVE 0 /* the reference's search: bit-exact */
+#define MPTC_SEARCH_LEARNED_INTER 1 /* inter: top-k proposals checked exactly;
+                                       intra: exhaustive for every block the
+                                       proposals leave */
+#define MPTC_SEARCH_LEARNED 2    /* inter and intra: top-k proposals checked
+                                    exactly; a block none passes stays unique */
+
+typedef struct {
+  int32_t batch;       /* rows per step */
+  int32_t steps;       /* optimizer steps */
+  float lr;            /* Adam step size */
+  float seed_weight;   /* weight of the endpoint-seed loss */
+  uint64_t seed;       /* shuffling seed */
+  float beta1, beta2, eps; /* Adam constants (0 = 0.9, 0.999, 1e-8) */
+  float lr_final;      /* step size of the last step, cosine decay (0 = constant lr) */
+} mptc_train_cfg;
+
+/* He-normal weights from `seed`, zero biases, Adam state cleared. */
+int32_t mptc_net_init(mptc_ctx* ctx, int32_t net, uint64_t seed);
+/* Host copies of the fp32 parameter vector; set clears the Adam state. */
+int32_t mptc_net_get_params(mptc_ctx* ctx, int32_t net, float* params_out);
+int32_t mptc_net_set_params(mptc_ctx* ctx, int32_t net, const float* params);
+/* Forward pass over n blocks: d_px n x 48 bytes, d_out n x 262 fp32 (device). */
+int32_t mptc_net_forward(mptc_ctx* ctx, int32_t net, const uint8_t* d_px, int32_t n, float* d_out);
+/* Loss and its gradient over n pairs (device inputs: pixels n x 48, targets
+ * n int32 logit indices, endpoints n x 6 bytes); grad_out: host,
+ * MPTC_NET_PARAMS floats. */
+int32_t mptc_net_gradients(mptc_ctx* ctx, int32_t net, const uint8_t* d_px, const int32_t* d_target,
+                           const uint8_t* d_eps, int32_t n, float seed_weight, float* grad_out,
+                           float* loss_out);
+/* cfg->steps Adam steps on batches drawn without replacement (reshuffled
+ * each epoch) from n_pairs device pairs; loss_out (host, optional) receives
+ * each step's batch loss. */
+int32_t mptc_net_train(mptc_ctx* ctx, int32_t net, const uint8_t* d_px, const int32_t* d_target,
+                       const uint8_t* d_eps, int32_t n_pairs, const mptc_train_cfg* cfg,
+                       float* loss_out);
+/* Training pairs from one encoded frame (device pointers: d_rgb, d_motion as
+ * written by an encode with this search_area, d_eps its eps output): every
+ * block reused from the window `net` scores appends (48 pixels, logit index,
+ * 6 endpoint bytes) at *d_count (device counter, advanced; pairs beyond
+ * capacity are dropped). */
+int32_t mptc_collect_pairs(mptc_ctx* ctx, int32_t net, const uint8_t* d_rgb, int32_t width,
+                           int32_t height, const uint8_t* d_motion, const uint8_t* d_eps,
+                           int32_t search_area, uint8_t* d_px, int32_t* d_target, uint8_t* d_eps_out,
+                           uint32_t* d_count, int32_t capacity);
+/* Search mode and proposals per block (top_k 1..16) for later encodes. */
+int32_t mptc_set_search_mode(mptc_ctx* ctx, int32_t mode, int32_t top_k);
+/* Replaces a network's scores by one fixed score per logit index (host,
+ * MPTC_NET_LOGITS floats; NULL returns to the network): the same offsets are
+ * proposed for every block. */
+int32_t mptc_set_prior(mptc_ctx* ctx, int32_t net, const float* scores);
 
 /* Seeded synthetic RGB8 frames (see mptc_amd/csrc/framegen.h): writes
  * n_frames frames starting at first_frame, each width*height*3 bytes.

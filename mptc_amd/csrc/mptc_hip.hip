@@ -17,12 +17,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <cmath>
 #include <mutex>
+#include <random>
 #include <string>
 #include <vector>
 
 #include "../../include/mptc.h"
-#include "batch_kernels.h"
+#include "mlp_kernels.h"
 #include "framegen.h"
 
 using namespace mptc;
@@ -100,12 +102,13 @@ int init_tables(int device) {
 
 enum KernelId {
   KF_ZERO, KF_FIT, KF_TABLE, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
-  KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_N
+  KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_N
 };
 const char* kKernelNames[KF_N] = {"zero",         "fit",          "table",       "inter",
                                   "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
                                   "intra_apply",  "intra_reeval", "intra_select", "commit",
-                                  "unique"};
+                                  "unique",       "net_input",   "net_forward", "propose_inter",
+                                  "top_intra",    "lintra_pass"};
 constexpr int kMaxChunks = 8;  // window-row chunks per block in the search kernels
 
 constexpr int kReevalGrid = 4096;  // workgroups of the re-evaluation pass (grid-stride)
@@ -118,6 +121,32 @@ struct Graphs {
     if (passes) (void)hipGraphExecDestroy(passes);
     if (suffix) (void)hipGraphExecDestroy(suffix);
     prefix = passes = suffix = nullptr;
+  }
+};
+
+// One proposal network: fp32 master parameters (layout of mlp_kernels.h),
+// their bf16 copy, Adam moments and gradient.
+struct NetState {
+  float* w = nullptr;
+  uint16_t* wb = nullptr;
+  float *m = nullptr, *v = nullptr, *g = nullptr;
+  int64_t t = 0;           // Adam steps taken
+  bool ready = false;      // parameters set
+  float* prior = nullptr;  // kNetLogits fixed scores
+  bool use_prior = false;
+};
+
+// Activations of a forward (and backward) pass over `rows` blocks.
+struct NetWork {
+  int rows = 0;
+  bool train = false;
+  uint16_t *X = nullptr, *H1 = nullptr, *H2 = nullptr, *dYb = nullptr, *dH2b = nullptr, *dH1b = nullptr;
+  float *Y = nullptr, *dYf = nullptr, *dH2f = nullptr, *dH1f = nullptr, *part = nullptr;
+  void release() {
+    void* b[] = {X, H1, H2, dYb, dH2b, dH1b, Y, dYf, dH2f, dH1f, part};
+    for (void* p : b)
+      if (p) (void)hipFree(p);
+    *this = NetWork();
   }
 };
 
@@ -157,6 +186,15 @@ struct mptc_ctx {
   int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0;
   Graphs graphs;
   int last_passes = 0, last_total = 0, last_intra = 0;
+  // learned proposals
+  int mode = MPTC_SEARCH_EXHAUSTIVE, top_k = 8;
+  NetState net[2];
+  NetWork nw;  // training and standalone forward passes
+  NetWork ew;  // the encode step's forward passes (max_total rows)
+  float* d_yt = nullptr;     // the step's logits, transposed: kNetOut x max_total
+  int16_t* d_prop = nullptr; // intra proposals, kMaxTopK per block
+  uint32_t* d_seen = nullptr;
+  int* d_errs = nullptr;
   // profiling
   bool prof = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -239,6 +277,156 @@ int dalloc(T** p, size_t n) {
 
 inline unsigned grid_for(int n, int wg) { return (unsigned)std::max(1, (n + wg - 1) / wg); }
 
+// ---------------------------------------------------------------------------
+// Proposal networks: buffers, GEMM launches, forward and backward passes.
+// ---------------------------------------------------------------------------
+int net_alloc(NetState& n) {
+  if (n.w) return MPTC_OK;
+  int rc;
+  if ((rc = dalloc(&n.w, kNetParams)) || (rc = dalloc(&n.wb, kNetParams)) || (rc = dalloc(&n.m, kNetParams)) ||
+      (rc = dalloc(&n.v, kNetParams)) || (rc = dalloc(&n.g, kNetParams)) || (rc = dalloc(&n.prior, kNetLogits)))
+    return rc;
+  return MPTC_OK;
+}
+
+void net_free(NetState& n) {
+  void* b[] = {n.w, n.wb, n.m, n.v, n.g, n.prior};
+  for (void* p : b)
+    if (p) (void)hipFree(p);
+  n = NetState();
+}
+
+int work_ensure(NetWork& w, int rows, bool train) {
+  if (w.rows >= rows && (w.train || !train)) return MPTC_OK;
+  w.release();
+  const size_t r = (size_t)rows;
+  int rc;
+  if ((rc = dalloc(&w.X, r * kNetIn)) || (rc = dalloc(&w.H1, r * kNetHid)) || (rc = dalloc(&w.H2, r * kNetHid)))
+    return rc;
+  if (train) {
+    const size_t parts = (r + 255) / 256;
+    if ((rc = dalloc(&w.Y, r * kNetOut)) || (rc = dalloc(&w.dYf, r * kNetOut)) || (rc = dalloc(&w.dYb, r * kNetOut)) ||
+        (rc = dalloc(&w.dH2f, r * kNetHid)) || (rc = dalloc(&w.dH2b, r * kNetHid)) ||
+        (rc = dalloc(&w.dH1f, r * kNetHid)) || (rc = dalloc(&w.dH1b, r * kNetHid)) ||
+        (rc = dalloc(&w.part, parts * kNetOut)))
+      return rc;
+  }
+  w.rows = rows;
+  w.train = train;
+  return MPTC_OK;
+}
+
+GemmEpi epi() {
+  GemmEpi e;
+  std::memset(&e, 0, sizeof(e));
+  return e;
+}
+
+// C = A * B (strided bf16 views) with the epilogue `ep`; rows are split over
+// launches so that the grid's y dimension stays within its limit.
+int gemm(mptc_ctx* c, hipStream_t s, int kid, int M, int N, int K, const uint16_t* A, long sam, long sak,
+         const uint16_t* B, long sbk, long sbn, GemmEpi ep) {
+  constexpr int kRowsPerLaunch = 65535 * 64;
+  for (int m0 = 0; m0 < M; m0 += kRowsPerLaunch) {
+    const int mc = std::min(M - m0, kRowsPerLaunch);
+    GemmEpi e = ep;
+    if (e.mask) e.mask += (long)m0 * e.ldm;
+    if (e.Cf) e.Cf += e.trans_f ? (long)m0 : (long)m0 * e.ldc;
+    if (e.Cb) e.Cb += (long)m0 * e.ldcb;
+    const uint16_t* a = A + (long)m0 * sam;
+    const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((mc + 63) / 64));
+    int rc = launch(c, s, kid, [&] {
+      hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, mc, N, K, a, sam, sak, B, sbk, sbn, e);
+    });
+    if (rc) return rc;
+  }
+  return MPTC_OK;
+}
+
+// Forward pass over w.X (rows blocks): H1, H2 in w, outputs to y (row-major
+// rows x kNetOut, or transposed kNetOut x rows when trans).
+int net_forward_q(mptc_ctx* c, hipStream_t s, const NetState& n, NetWork& w, int rows, float* y, bool trans) {
+  int rc;
+  GemmEpi e = epi();
+  e.bias = n.w + kOffB1;
+  e.relu = 1;
+  e.Cb = w.H1;
+  e.ldcb = kNetHid;
+  if ((rc = gemm(c, s, KF_NET_FWD, rows, kNetHid, kNetIn, w.X, kNetIn, 1, n.wb + kOffW1, kNetHid, 1, e))) return rc;
+  e.bias = n.w + kOffB2;
+  e.Cb = w.H2;
+  if ((rc = gemm(c, s, KF_NET_FWD, rows, kNetHid, kNetHid, w.H1, kNetHid, 1, n.wb + kOffW2, kNetHid, 1, e)))
+    return rc;
+  e = epi();
+  e.bias = n.w + kOffB3;
+  e.Cf = y;
+  e.trans_f = trans ? 1 : 0;
+  e.ldc = trans ? rows : kNetOut;
+  return gemm(c, s, KF_NET_FWD, rows, kNetOut, kNetHid, w.H2, kNetHid, 1, n.wb + kOffW3, kNetOut, 1, e);
+}
+
+int colsum_q(mptc_ctx* c, hipStream_t s, const float* X, int rows, int cols, long ld, float* part, float* out) {
+  const int parts = (rows + 255) / 256;
+  return launch(c, s, KF_NET_FWD, [&] {
+    hipLaunchKernelGGL(k_colsum_part, dim3(grid_for(cols, 256), parts), dim3(256), 0, s, X, rows, cols, ld, part);
+    hipLaunchKernelGGL(k_colsum_final, dim3(grid_for(cols, 256)), dim3(256), 0, s, part, parts, cols, out);
+  });
+}
+
+// Loss and gradient of a forward pass over w (rows = the batch): n.g receives
+// d(mean loss)/d(parameters), *loss (device) the mean loss.
+int net_backward_q(mptc_ctx* c, hipStream_t s, NetState& n, NetWork& w, int rows, const int32_t* target,
+                   const uint8_t* eps, const int32_t* idx, float seed_w, float* loss) {
+  int rc;
+  if ((rc = launch(c, s, KF_NET_FWD, [&] {
+         hipLaunchKernelGGL(k_net_loss, dim3(grid_for(rows, 4)), dim3(256), 0, s, w.Y, rows, target, eps, idx, seed_w,
+                            w.dYf, w.dYb, loss);
+       })))
+    return rc;
+  GemmEpi e = epi();
+  // layer 3: dW3 = H2^T dY, db3 = sum dY, dH2 = (dY W3^T) * [H2 > 0]
+  e.Cf = n.g + kOffW3;
+  e.ldc = kNetOut;
+  if ((rc = gemm(c, s, KF_NET_FWD, kNetHid, kNetOut, rows, w.H2, 1, kNetHid, w.dYb, kNetOut, 1, e))) return rc;
+  if ((rc = colsum_q(c, s, w.dYf, rows, kNetOut, kNetOut, w.part, n.g + kOffB3))) return rc;
+  e = epi();
+  e.mask = w.H2;
+  e.ldm = kNetHid;
+  e.Cf = w.dH2f;
+  e.ldc = kNetHid;
+  e.Cb = w.dH2b;
+  e.ldcb = kNetHid;
+  if ((rc = gemm(c, s, KF_NET_FWD, rows, kNetHid, kNetOut, w.dYb, kNetOut, 1, n.wb + kOffW3, 1, kNetOut, e)))
+    return rc;
+  // layer 2
+  e = epi();
+  e.Cf = n.g + kOffW2;
+  e.ldc = kNetHid;
+  if ((rc = gemm(c, s, KF_NET_FWD, kNetHid, kNetHid, rows, w.H1, 1, kNetHid, w.dH2b, kNetHid, 1, e))) return rc;
+  if ((rc = colsum_q(c, s, w.dH2f, rows, kNetHid, kNetHid, w.part, n.g + kOffB2))) return rc;
+  e = epi();
+  e.mask = w.H1;
+  e.ldm = kNetHid;
+  e.Cf = w.dH1f;
+  e.ldc = kNetHid;
+  e.Cb = w.dH1b;
+  e.ldcb = kNetHid;
+  if ((rc = gemm(c, s, KF_NET_FWD, rows, kNetHid, kNetHid, w.dH2b, kNetHid, 1, n.wb + kOffW2, 1, kNetHid, e)))
+    return rc;
+  // layer 1
+  e = epi();
+  e.Cf = n.g + kOffW1;
+  e.ldc = kNetHid;
+  if ((rc = gemm(c, s, KF_NET_FWD, kNetIn, kNetHid, rows, w.X, 1, kNetIn, w.dH1b, kNetHid, 1, e))) return rc;
+  return colsum_q(c, s, w.dH1f, rows, kNetHid, kNetHid, w.part, n.g + kOffB1);
+}
+
+int net_input_q(mptc_ctx* c, hipStream_t s, const uint8_t* px, const int32_t* idx, int rows, uint16_t* X) {
+  return launch(c, s, KF_NET_INPUT, [&] {
+    hipLaunchKernelGGL(k_net_input, dim3(grid_for(rows * kNetIn, 256)), dim3(256), 0, s, px, idx, rows, X);
+  });
+}
+
 // One frame of a batch, host view.
 struct FrameIn {
   const uint8_t* rgb;
@@ -313,12 +501,34 @@ int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
   return MPTC_OK;
 }
 
+// Scores of network `net` for every block of the batch into d_yt (nothing
+// when a fixed prior replaces the network).
+int learned_scores_q(mptc_ctx* c, hipStream_t s, int net) {
+  const NetState& n = c->net[net];
+  if (n.use_prior) return MPTC_OK;
+  const int total = c->total;
+  int rc = launch(c, s, KF_NET_INPUT, [&] {
+    hipLaunchKernelGGL(k_frame_net_input, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total,
+                       c->ew.X);
+  });
+  if (rc) return rc;
+  return net_forward_q(c, s, n, c->ew, total, c->d_yt, true);
+}
+
+int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first) {
+  const int total = c->total, S = c->cfg.search_area, thr = c->cfg.err_threshold;
+  return launch(c, s, KF_LPASS, [&] {
+    hipLaunchKernelGGL(k_lintra_pass, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total, S, thr,
+                       c->top_k, c->d_prop, c->d_own, c->d_need, c->d_cw, c->d_seen, c->d_errs, c->d_intra,
+                       c->d_stats, par, first);
+  });
+}
+
 // The three launch sequences of a step.  `fit` = false starts from
 // caller-provided fitted blocks already in d_own.
 int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
   const int total = c->total, wg = c->tune.wg_size, tw = c->tune.tile_w;
   const int S = c->cfg.search_area, thr = c->cfg.err_threshold;
-  const unsigned gw = grid_for(c->n_wtiles * 64, wg);
   int rc;
   if ((rc = launch(c, s, KF_ZERO, [&] {
          hipLaunchKernelGGL(k_zero_u32, dim3(grid_for((int)(sizeof(DevStats) / 4), 256)), dim3(256), 0, s,
@@ -338,21 +548,44 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
     return rc;
   const int K = chunks_for(S, c->tune.win_chunks);
   const unsigned gwk = grid_for(c->n_wtiles * K * 64, wg);
-  if ((rc = launch(c, s, KF_INTER, [&] {
-         hipLaunchKernelGGL(k_inter, dim3(gwk), dim3(wg), 0, s, c->d_fr, c->d_tsid, c->n_wtiles, tw, K, S, c->d_own,
-                            c->d_rw, c->d_rtab, c->d_summ, c->nbs, c->d_stats);
-       })))
-    return rc;
-  if ((rc = launch(c, s, KF_INTER_COMBINE, [&] {
-         hipLaunchKernelGGL(k_inter_combine, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total, K,
-                            S, thr, c->d_rw, c->d_summ, c->nbs, c->d_inter);
-       })))
-    return rc;
+  if (c->mode == MPTC_SEARCH_EXHAUSTIVE) {
+    if ((rc = launch(c, s, KF_INTER, [&] {
+           hipLaunchKernelGGL(k_inter, dim3(gwk), dim3(wg), 0, s, c->d_fr, c->d_tsid, c->n_wtiles, tw, K, S,
+                              c->d_own, c->d_rw, c->d_rtab, c->d_summ, c->nbs, c->d_stats);
+         })))
+      return rc;
+    if ((rc = launch(c, s, KF_INTER_COMBINE, [&] {
+           hipLaunchKernelGGL(k_inter_combine, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
+                              K, S, thr, c->d_rw, c->d_summ, c->nbs, c->d_inter);
+         })))
+      return rc;
+  } else {
+    if ((rc = learned_scores_q(c, s, MPTC_NET_INTER))) return rc;
+    const float* ys = c->net[0].use_prior ? c->net[0].prior : c->d_yt;
+    const long ys_l = c->net[0].use_prior ? 1 : total, ys_g = c->net[0].use_prior ? 0 : 1;
+    if ((rc = launch(c, s, KF_PROPOSE, [&] {
+           hipLaunchKernelGGL(k_propose_inter, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
+                              S, thr, c->top_k, ys, ys_l, ys_g, c->d_own, c->d_rw, c->d_rtab, c->d_inter,
+                              c->d_stats);
+         })))
+      return rc;
+  }
   if ((rc = launch(c, s, KF_PREP, [&] {
          hipLaunchKernelGGL(k_intra_prep, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
                             c->d_own, c->d_inter, c->d_need, c->d_cw, c->d_ctab, c->d_dirty, c->d_orig);
        })))
     return rc;
+  if (c->mode == MPTC_SEARCH_LEARNED) {
+    if ((rc = learned_scores_q(c, s, MPTC_NET_INTRA))) return rc;
+    const float* ys = c->net[1].use_prior ? c->net[1].prior : c->d_yt;
+    const long ys_l = c->net[1].use_prior ? 1 : total, ys_g = c->net[1].use_prior ? 0 : 1;
+    if ((rc = launch(c, s, KF_TOP_INTRA, [&] {
+           hipLaunchKernelGGL(k_top_intra, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total, S,
+                              c->top_k, ys, ys_l, ys_g, c->d_need, c->d_prop);
+         })))
+      return rc;
+    return lintra_pass_q(c, s, 0, 1);
+  }
   if ((rc = launch(c, s, KF_DENSE, [&] {
          hipLaunchKernelGGL(k_intra_dense, dim3(gwk), dim3(wg), 0, s, c->d_fr, c->d_tsid, c->n_wtiles, tw, K, S,
                             c->d_own, c->d_need, c->d_cw, c->d_ctab, c->d_orig, c->d_E, c->nbs, c->d_summ, c->nbs,
@@ -379,6 +612,12 @@ int queue_passes(mptc_ctx* c, hipStream_t s) {
   const int S = c->cfg.search_area, thr = c->cfg.err_threshold, total = c->total;
   const int np = passes_per_batch(c);
   int rc;
+  if (c->mode == MPTC_SEARCH_LEARNED) {
+    // parities 1, 0, ..., 0: the batch's last pass counts into n_changed[0]
+    for (int p = 0; p < np; ++p)
+      if ((rc = lintra_pass_q(c, s, (p + 1) & 1, 0))) return rc;
+    return MPTC_OK;
+  }
   for (int p = 0; p < np; ++p) {
     const int par = p & 1;
     if ((rc = launch(c, s, KF_REEVAL, [&] {
@@ -568,6 +807,13 @@ void mptc_destroy(mptc_ctx* c) {
                   c->d_orig, c->d_chg,   c->d_E, c->d_summ,     c->d_stats, c->d_rgb,    c->d_ref,
                   c->d_phys, c->d_motion, c->d_eps,   c->d_kind,  c->d_unique, c->d_nunique};
   for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  net_free(c->net[0]);
+  net_free(c->net[1]);
+  c->nw.release();
+  c->ew.release();
+  void* lbufs[] = {c->d_yt, c->d_prop, c->d_seen, c->d_errs};
+  for (void* p : lbufs)
     if (p) (void)hipFree(p);
   if (c->h_stats) (void)hipHostFree(c->h_stats);
   if (c->h_pass) (void)hipHostFree(c->h_pass);
@@ -768,6 +1014,223 @@ int32_t mptc_profile_read(mptc_ctx* c, const char* name, int64_t* launches, doub
       return MPTC_OK;
     }
   return fail(MPTC_EINVAL, std::string("unknown kernel name: ") + name);
+}
+
+// ---------------------------------------------------------------------------
+// Learned proposals
+// ---------------------------------------------------------------------------
+static int check_net(mptc_ctx* c, int net) {
+  if (!c) return fail(MPTC_EINVAL, "null context");
+  if (net != MPTC_NET_INTER && net != MPTC_NET_INTRA) return fail(MPTC_EINVAL, "net must be 0 (inter) or 1 (intra)");
+  HIP_TRY(hipSetDevice(c->cfg.device));
+  return net_alloc(c->net[net]);
+}
+
+static int upload_params(mptc_ctx* c, NetState& n, const float* params) {
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemcpyAsync(n.w, params, sizeof(float) * kNetParams, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(kNetParams, 256)), dim3(256), 0, s, n.w, kNetParams, n.wb);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemsetAsync(n.m, 0, sizeof(float) * kNetParams, s));
+  HIP_TRY(hipMemsetAsync(n.v, 0, sizeof(float) * kNetParams, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  n.t = 0;
+  n.ready = true;
+  return MPTC_OK;
+}
+
+int32_t mptc_net_init(mptc_ctx* c, int32_t net, uint64_t seed) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  std::vector<float> p(kNetParams, 0.0f);
+  std::mt19937_64 rng(seed);
+  auto fill = [&](int off, int fan_in, int count, float gain) {
+    std::normal_distribution<float> nd(0.0f, std::sqrt(gain / (float)fan_in));
+    for (int i = 0; i < count; ++i) p[off + i] = nd(rng);
+  };
+  fill(kOffW1, kNetIn, kNetIn * kNetHid, 2.0f);
+  fill(kOffW2, kNetHid, kNetHid * kNetHid, 2.0f);
+  fill(kOffW3, kNetHid, kNetHid * kNetOut, 1.0f);
+  return upload_params(c, c->net[net], p.data());
+}
+
+int32_t mptc_net_get_params(mptc_ctx* c, int32_t net, float* out) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  if (!out) return fail(MPTC_EINVAL, "null argument");
+  if (!c->net[net].ready) return fail(MPTC_ESTATE, "network parameters not set");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(out, c->net[net].w, sizeof(float) * kNetParams, hipMemcpyDeviceToHost));
+  return MPTC_OK;
+}
+
+int32_t mptc_net_set_params(mptc_ctx* c, int32_t net, const float* params) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  if (!params) return fail(MPTC_EINVAL, "null argument");
+  return upload_params(c, c->net[net], params);
+}
+
+int32_t mptc_net_forward(mptc_ctx* c, int32_t net, const uint8_t* d_px, int32_t n, float* d_out) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  if (!d_px || !d_out || n < 1) return fail(MPTC_EINVAL, "bad argument");
+  if (!c->net[net].ready) return fail(MPTC_ESTATE, "network parameters not set");
+  hipStream_t s = c->stream;
+  if ((rc = work_ensure(c->nw, n, false)) || (rc = net_input_q(c, s, d_px, nullptr, n, c->nw.X)) ||
+      (rc = net_forward_q(c, s, c->net[net], c->nw, n, d_out, false)))
+    return rc;
+  HIP_TRY(hipStreamSynchronize(s));
+  return MPTC_OK;
+}
+
+int32_t mptc_net_gradients(mptc_ctx* c, int32_t net, const uint8_t* d_px, const int32_t* d_target,
+                           const uint8_t* d_eps, int32_t n, float seed_w, float* grad_out, float* loss_out) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  if (!d_px || !d_target || !d_eps || !grad_out || n < 1) return fail(MPTC_EINVAL, "bad argument");
+  NetState& ns = c->net[net];
+  if (!ns.ready) return fail(MPTC_ESTATE, "network parameters not set");
+  hipStream_t s = c->stream;
+  float* d_loss = nullptr;
+  if ((rc = dalloc(&d_loss, 1))) return rc;
+  HIP_TRY(hipMemsetAsync(d_loss, 0, sizeof(float), s));
+  if ((rc = work_ensure(c->nw, n, true)) || (rc = net_input_q(c, s, d_px, nullptr, n, c->nw.X)) ||
+      (rc = net_forward_q(c, s, ns, c->nw, n, c->nw.Y, false)) ||
+      (rc = net_backward_q(c, s, ns, c->nw, n, d_target, d_eps, nullptr, seed_w, d_loss))) {
+    (void)hipFree(d_loss);
+    return rc;
+  }
+  float loss = 0.0f;
+  const hipError_t e1 = hipMemcpyAsync(grad_out, ns.g, sizeof(float) * kNetParams, hipMemcpyDeviceToHost, s);
+  const hipError_t e2 = hipMemcpyAsync(&loss, d_loss, sizeof(float), hipMemcpyDeviceToHost, s);
+  const hipError_t e3 = hipStreamSynchronize(s);
+  (void)hipFree(d_loss);
+  HIP_TRY(e1);
+  HIP_TRY(e2);
+  HIP_TRY(e3);
+  if (loss_out) *loss_out = loss;
+  return MPTC_OK;
+}
+
+int32_t mptc_net_train(mptc_ctx* c, int32_t net, const uint8_t* d_px, const int32_t* d_target, const uint8_t* d_eps,
+                       int32_t n_pairs, const mptc_train_cfg* cfg, float* loss_out) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  if (!d_px || !d_target || !d_eps || !cfg || n_pairs < 1) return fail(MPTC_EINVAL, "bad argument");
+  if (cfg->batch < 1 || cfg->batch > (1 << 22) || cfg->steps < 1 || !(cfg->lr > 0.0f))
+    return fail(MPTC_EINVAL, "bad training configuration");
+  NetState& ns = c->net[net];
+  if (!ns.ready) return fail(MPTC_ESTATE, "network parameters not set");
+  const int batch = std::min(cfg->batch, n_pairs);
+  const float b1 = cfg->beta1 > 0.0f ? cfg->beta1 : 0.9f, b2 = cfg->beta2 > 0.0f ? cfg->beta2 : 0.999f;
+  const float epsv = cfg->eps > 0.0f ? cfg->eps : 1e-8f;
+  hipStream_t s = c->stream;
+  if ((rc = work_ensure(c->nw, batch, true))) return rc;
+  int32_t* d_perm = nullptr;
+  float* d_losses = nullptr;
+  auto done = [&](int code) {
+    (void)hipStreamSynchronize(s);
+    if (d_perm) (void)hipFree(d_perm);
+    if (d_losses) (void)hipFree(d_losses);
+    return code;
+  };
+  if ((rc = dalloc(&d_perm, n_pairs)) || (rc = dalloc(&d_losses, cfg->steps))) return done(rc);
+  if (hipMemsetAsync(d_losses, 0, sizeof(float) * cfg->steps, s) != hipSuccess)
+    return done(fail(MPTC_EHIP, "hipMemsetAsync failed"));
+  std::vector<int32_t> perm(n_pairs);
+  for (int i = 0; i < n_pairs; ++i) perm[i] = i;
+  std::mt19937_64 rng(cfg->seed);
+  int pos = n_pairs;
+  for (int step = 0; step < cfg->steps; ++step) {
+    if (pos + batch > n_pairs) {
+      // the previous epoch's upload must be done before the host order changes
+      if (hipStreamSynchronize(s) != hipSuccess) return done(fail(MPTC_EHIP, "hipStreamSynchronize failed"));
+      std::shuffle(perm.begin(), perm.end(), rng);
+      if (hipMemcpyAsync(d_perm, perm.data(), sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, s) != hipSuccess)
+        return done(fail(MPTC_EHIP, "hipMemcpyAsync failed"));
+      pos = 0;
+    }
+    const int32_t* idx = d_perm + pos;
+    pos += batch;
+    if ((rc = net_input_q(c, s, d_px, idx, batch, c->nw.X)) ||
+        (rc = net_forward_q(c, s, ns, c->nw, batch, c->nw.Y, false)) ||
+        (rc = net_backward_q(c, s, ns, c->nw, batch, d_target, d_eps, idx, cfg->seed_weight, d_losses + step)))
+      return done(rc);
+    float lr = cfg->lr;
+    if (cfg->lr_final > 0.0f && cfg->steps > 1) {
+      const double u = (double)step / (double)(cfg->steps - 1);
+      lr = (float)(cfg->lr_final + 0.5 * (cfg->lr - cfg->lr_final) * (1.0 + std::cos(3.141592653589793 * u)));
+    }
+    ns.t += 1;
+    const float c1 = (float)(1.0 - std::pow((double)b1, (double)ns.t));
+    const float c2 = (float)(1.0 - std::pow((double)b2, (double)ns.t));
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(kNetParams, 256)), dim3(256), 0, s, ns.w, ns.g, ns.m, ns.v, ns.wb,
+                       kNetParams, lr, b1, b2, c1, c2, epsv);
+    if (hipGetLastError() != hipSuccess) return done(fail(MPTC_EHIP, "k_adam launch failed"));
+  }
+  if (loss_out &&
+      hipMemcpyAsync(loss_out, d_losses, sizeof(float) * cfg->steps, hipMemcpyDeviceToHost, s) != hipSuccess)
+    return done(fail(MPTC_EHIP, "hipMemcpyAsync failed"));
+  return done(MPTC_OK);
+}
+
+int32_t mptc_collect_pairs(mptc_ctx* c, int32_t net, const uint8_t* d_rgb, int32_t width, int32_t height,
+                           const uint8_t* d_motion, const uint8_t* d_eps, int32_t S, uint8_t* d_px, int32_t* d_target,
+                           uint8_t* d_eps_out, uint32_t* d_count, int32_t capacity) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  if (!d_rgb || !d_motion || !d_eps || !d_px || !d_target || !d_eps_out || !d_count || capacity < 0)
+    return fail(MPTC_EINVAL, "bad argument");
+  if (width <= 0 || height <= 0 || (width & 3) || (height & 3))
+    return fail(MPTC_EINVAL, "width and height must be positive multiples of 4");
+  if (S < 1 || S > kMaxS) return fail(MPTC_EINVAL, "search_area must be in 1..63");
+  const int bw = width / 4, nb = bw * (height / 4);
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_collect_pairs, dim3(grid_for(nb, 256)), dim3(256), 0, s, d_rgb, width, bw, nb, d_motion, d_eps,
+                     S, net, d_px, d_target, d_eps_out, d_count, capacity);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  return MPTC_OK;
+}
+
+int32_t mptc_set_search_mode(mptc_ctx* c, int32_t mode, int32_t top_k) {
+  if (!c) return fail(MPTC_EINVAL, "null context");
+  if (mode < MPTC_SEARCH_EXHAUSTIVE || mode > MPTC_SEARCH_LEARNED) return fail(MPTC_EINVAL, "unknown search mode");
+  if (top_k < 1 || top_k > kMaxTopK) return fail(MPTC_EINVAL, "top_k must be in 1..16");
+  HIP_TRY(hipSetDevice(c->cfg.device));
+  if (mode != MPTC_SEARCH_EXHAUSTIVE) {
+    const int needed = mode == MPTC_SEARCH_LEARNED ? 2 : 1;
+    for (int i = 0; i < needed; ++i)
+      if (!c->net[i].ready && !c->net[i].use_prior)
+        return fail(MPTC_ESTATE, "learned search needs trained network parameters or a prior");
+    int rc;
+    const size_t nb = (size_t)c->max_total;
+    if ((rc = work_ensure(c->ew, c->max_total, false))) return rc;
+    if (!c->d_yt && ((rc = dalloc(&c->d_yt, nb * kNetOut)) || (rc = dalloc(&c->d_prop, nb * kMaxTopK)) ||
+                     (rc = dalloc(&c->d_seen, nb * kMaxTopK)) || (rc = dalloc(&c->d_errs, nb * kMaxTopK))))
+      return rc;
+  }
+  if (mode != c->mode || top_k != c->top_k) c->graphs.destroy();
+  c->mode = mode;
+  c->top_k = top_k;
+  return MPTC_OK;
+}
+
+int32_t mptc_set_prior(mptc_ctx* c, int32_t net, const float* scores) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  NetState& n = c->net[net];
+  if (scores) {
+    HIP_TRY(hipMemcpy(n.prior, scores, sizeof(float) * kNetLogits, hipMemcpyHostToDevice));
+    n.use_prior = true;
+  } else {
+    if (c->mode == MPTC_SEARCH_LEARNED || (c->mode == MPTC_SEARCH_LEARNED_INTER && net == MPTC_NET_INTER))
+      if (!n.ready) return fail(MPTC_ESTATE, "network parameters not set");
+    n.use_prior = false;
+  }
+  c->graphs.destroy();
+  return MPTC_OK;
 }
 
 int32_t mptc_synth_frames(const void* cfg, int32_t first, int32_t n, uint8_t* rgb_out) {

@@ -55,6 +55,16 @@ class FrameStats(C.Structure):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
 
 
+class TrainCfg(C.Structure):
+    _fields_ = [("batch", C.c_int32), ("steps", C.c_int32), ("lr", C.c_float),
+                ("seed_weight", C.c_float), ("seed", C.c_uint64), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("lr_final", C.c_float)]
+
+
+NET_INTER, NET_INTRA = 0, 1
+NET_IN, NET_HIDDEN, NET_OUT, NET_LOGITS, NET_PARAMS = 48, 256, 262, 256, 145670
+SEARCH_EXHAUSTIVE, SEARCH_LEARNED_INTER, SEARCH_LEARNED = 0, 1, 2
+
 _u8p = C.POINTER(C.c_uint8)
 _u32p = C.POINTER(C.c_uint32)
 _u64p = C.POINTER(C.c_uint64)
@@ -96,6 +106,17 @@ def load() -> C.CDLL:
     L.mptc_set_tuning.argtypes = [_vp, C.POINTER(Tuning)]
     L.mptc_autotune.argtypes = [_vp, C.c_int32, C.POINTER(FrameIO), C.c_int32, C.POINTER(Tuning),
                                 C.POINTER(C.c_double)]
+    L.mptc_net_init.argtypes = [_vp, C.c_int32, C.c_uint64]
+    L.mptc_net_get_params.argtypes = [_vp, C.c_int32, _vp]
+    L.mptc_net_set_params.argtypes = [_vp, C.c_int32, _vp]
+    L.mptc_net_forward.argtypes = [_vp, C.c_int32, _vp, C.c_int32, _vp]
+    L.mptc_net_gradients.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.c_float, _vp,
+                                     C.POINTER(C.c_float)]
+    L.mptc_net_train.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.POINTER(TrainCfg), _vp]
+    L.mptc_collect_pairs.argtypes = [_vp, C.c_int32, _vp, C.c_int32, C.c_int32, _vp, _vp, C.c_int32,
+                                     _vp, _vp, _vp, _vp, C.c_int32]
+    L.mptc_set_search_mode.argtypes = [_vp, C.c_int32, C.c_int32]
+    L.mptc_set_prior.argtypes = [_vp, C.c_int32, _vp]
     for name in header_functions():
         if name not in ("mptc_destroy", "mptc_last_error"):
             getattr(L, name).restype = C.c_int32
@@ -242,6 +263,55 @@ class Encoder:
         n, ms = C.c_int64(), C.c_double()
         _check(self.L.mptc_profile_read(self.h, name.encode(), C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    # ---- learned proposals ----
+    def net_init(self, net: int, seed: int):
+        _check(self.L.mptc_net_init(self.h, net, seed))
+
+    def net_params(self, net: int) -> np.ndarray:
+        out = np.zeros(NET_PARAMS, np.float32)
+        _check(self.L.mptc_net_get_params(self.h, net, out.ctypes.data))
+        return out
+
+    def net_set_params(self, net: int, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == NET_PARAMS
+        _check(self.L.mptc_net_set_params(self.h, net, p.ctypes.data))
+
+    def net_forward(self, net: int, d_px, n: int, d_out):
+        _check(self.L.mptc_net_forward(self.h, net, _ptr(d_px), n, _ptr(d_out)))
+
+    def net_gradients(self, net: int, d_px, d_target, d_eps, n: int, seed_weight: float):
+        g = np.zeros(NET_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(self.L.mptc_net_gradients(self.h, net, _ptr(d_px), _ptr(d_target), _ptr(d_eps), n,
+                                         seed_weight, g.ctypes.data, C.byref(loss)))
+        return g, loss.value
+
+    def net_train(self, net: int, d_px, d_target, d_eps, n_pairs: int, batch=4096, steps=1000,
+                  lr=1e-3, seed_weight=1.0, seed=1, lr_final=0.0) -> np.ndarray:
+        cfg = TrainCfg(batch, steps, lr, seed_weight, seed, 0.0, 0.0, 0.0, lr_final)
+        losses = np.zeros(steps, np.float32)
+        _check(self.L.mptc_net_train(self.h, net, _ptr(d_px), _ptr(d_target), _ptr(d_eps), n_pairs,
+                                     C.byref(cfg), losses.ctypes.data))
+        return losses
+
+    def collect_pairs(self, net: int, d_rgb, w: int, h: int, d_motion, d_eps, search_area: int,
+                      d_px, d_target, d_eps_out, d_count, capacity: int):
+        _check(self.L.mptc_collect_pairs(self.h, net, _ptr(d_rgb), w, h, _ptr(d_motion), _ptr(d_eps),
+                                         search_area, _ptr(d_px), _ptr(d_target), _ptr(d_eps_out),
+                                         _ptr(d_count), capacity))
+
+    def set_search_mode(self, mode: int, top_k: int = 8):
+        _check(self.L.mptc_set_search_mode(self.h, mode, top_k))
+
+    def set_prior(self, net: int, scores: np.ndarray | None):
+        if scores is None:
+            _check(self.L.mptc_set_prior(self.h, net, None))
+        else:
+            p = np.ascontiguousarray(scores, np.float32)
+            assert p.size == NET_LOGITS
+            _check(self.L.mptc_set_prior(self.h, net, p.ctypes.data))
 
     def encode_sequence(self, frames: np.ndarray, intra_interval: int):
         """The reference frame schedule (codec/codec.cpp:1379-1508, entropy stage

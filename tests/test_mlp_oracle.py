@@ -1,0 +1,96 @@
+"""CPU checks of the network restatement (oracle/mlp.py): bf16 rounding
+against torch's, the backward formulas against finite differences of the
+loss, Adam against its closed form after one step."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp
+
+
+def test_bf16_rounding_matches_torch():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([
+        rng.normal(0, 1, 20000).astype(np.float32),
+        rng.normal(0, 1e-30, 2000).astype(np.float32),
+        (rng.integers(0, 2**31, 5000, dtype=np.int64).astype(np.uint32)).view(np.float32),
+        # exact ties: low 16 bits 0x8000, even and odd upper halves
+        (np.arange(1000, dtype=np.uint32) << 16 | 0x8000).view(np.float32),
+        np.array([0.0, -0.0, np.inf, -np.inf, 3.0e38, -3.0e38, 1e-45], np.float32),
+    ])
+    x = x[np.isfinite(x) | np.isinf(x)]
+    ours = mlp.bf16(x)
+    ref = torch.from_numpy(x.copy()).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(ours.view(np.uint32), ref.view(np.uint32))
+    nan = mlp.bf16(np.array([np.nan], np.float32))
+    assert np.isnan(nan[0])
+
+
+def _fd_setup(seed=3, n=6):
+    rng = np.random.default_rng(seed)
+    p = mlp.he_params(seed).astype(np.float64)
+    x = rng.uniform(0, 1, (n, mlp.N_IN))
+    t = rng.integers(0, mlp.N_LOGITS, n)
+    eps = rng.integers(0, 256, (n, mlp.N_SEEDS))
+    return p, x, t, eps
+
+
+@pytest.mark.parametrize("block", ["W1", "b1", "W2", "b2", "W3", "b3"])
+def test_backward_matches_finite_differences(block):
+    p, x, t, eps = _fd_setup()
+    sw = 0.7
+    _, g = mlp.loss_grad(p, x, t, eps, sw, exact=True)
+    offs = {"W1": (mlp.OFF_W1, mlp.OFF_B1), "b1": (mlp.OFF_B1, mlp.OFF_W2),
+            "W2": (mlp.OFF_W2, mlp.OFF_B2), "b2": (mlp.OFF_B2, mlp.OFF_W3),
+            "W3": (mlp.OFF_W3, mlp.OFF_B3), "b3": (mlp.OFF_B3, mlp.N_PARAMS)}[block]
+    rng = np.random.default_rng(11)
+    idx = rng.choice(np.arange(*offs), size=min(40, offs[1] - offs[0]), replace=False)
+    # include the seed outputs of the last layer
+    if block in ("W3", "b3"):
+        extra = [offs[0] + mlp.N_LOGITS + 2] if block == "b3" else [offs[0] + 5 * mlp.N_OUT + 258]
+        idx = np.concatenate([idx, extra])
+    h = 1e-6
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        lp, _ = mlp.loss_grad(pp, x, t, eps, sw, exact=True)
+        lm, _ = mlp.loss_grad(pm, x, t, eps, sw, exact=True)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (block, i, fd, g[i])
+
+
+def test_gradient_is_not_trivially_zero():
+    p, x, t, eps = _fd_setup()
+    _, g = mlp.loss_grad(p, x, t, eps, 0.5, exact=True)
+    for a, b in [(mlp.OFF_W1, mlp.OFF_B1), (mlp.OFF_W2, mlp.OFF_B2), (mlp.OFF_W3, mlp.OFF_B3)]:
+        assert np.abs(g[a:b]).max() > 1e-4
+
+
+def test_bf16_path_close_to_exact():
+    rng = np.random.default_rng(5)
+    p = mlp.he_params(5)
+    px = rng.integers(0, 256, (64, mlp.N_IN)).astype(np.uint8)
+    t = rng.integers(0, 256, 64)
+    eps = rng.integers(0, 256, (64, 6))
+    l0, g0 = mlp.loss_grad(p, px, t, eps, 1.0, exact=False)
+    l1, g1 = mlp.loss_grad(p.astype(np.float64), px.astype(np.float64) / 255.0, t, eps, 1.0, exact=True)
+    assert abs(l0 - l1) < 0.05 * abs(l1)
+    assert np.linalg.norm(g0 - g1) < 0.05 * np.linalg.norm(g1)
+
+
+def test_loss_of_uniform_logits_is_log256():
+    p = np.zeros(mlp.N_PARAMS)
+    px = np.zeros((4, mlp.N_IN), np.uint8)
+    eps = np.zeros((4, 6))
+    loss, g = mlp.loss_grad(p, px, np.array([0, 5, 255, 17]), eps, 3.0)
+    assert abs(loss - np.log(256.0)) < 1e-12
+
+
+def test_adam_first_step_is_signed_lr():
+    rng = np.random.default_rng(2)
+    w = rng.normal(size=100)
+    g = rng.normal(size=100)
+    w1, m, v = mlp.adam_step(w, g, np.zeros(100), np.zeros(100), 1, 0.01, eps=0.0)
+    assert np.allclose(w1, w - 0.01 * np.sign(g))
+    assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)

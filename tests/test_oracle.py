@@ -146,3 +146,33 @@ def test_generator_is_stable(orc):
         from tests.helpers import sha
         assert sha(frames[0]) == fx["frames"][0]["input_sha"]
         assert sha(frames[1]) == fx["frames"][1]["input_sha"]
+
+
+def test_reuse_checker_accepts_the_oracle_and_catches_planted_mistakes(orc):
+    """tests/helpers.check_reuse_frame (the check of the learned search modes)
+    passes the oracle's own exhaustive encode and fails it when one reuse is
+    planted wrong (another word, endpoints off by one step, a broken palette)."""
+    from tests.helpers import check_reuse_frame
+    spec = SEQUENCES["c1_s8_t8"]
+    frames = orc.synth_frames(synth_cfg(spec), 3)
+    S, thr = spec["S"], spec["thr"]
+    prev = None
+    for f, rgb in enumerate(frames):
+        own = orc.fit_frame(rgb)
+        r = orc.reencode(rgb, own, S, thr, f == 0, prev)
+        ni, na = check_reuse_frame(orc, rgb, own, r["phys"], r["motion"], r["unique"], r["eps"], f == 0, prev, S, thr)
+        assert ni + na == int((r["kind"] > 0).sum()) > 0
+        if f == 2:
+            reused = np.flatnonzero(r["kind"] > 0)
+            k = int(reused[len(reused) // 2])
+            for plant in ("word", "endpoint", "palette"):
+                phys, unique = r["phys"].copy(), r["unique"].copy()
+                if plant == "word":
+                    phys[k] ^= np.uint64(1 << 40)
+                elif plant == "endpoint":
+                    phys[k] ^= np.uint64(1)
+                else:
+                    unique[0] ^= np.uint32(4)
+                with pytest.raises(AssertionError):
+                    check_reuse_frame(orc, rgb, own, phys, r["motion"], unique, r["eps"], False, prev, S, thr)
+        prev = r["phys"]
