@@ -157,8 +157,8 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
  * accumulated launch count and milliseconds of kernel `name` ("fit", "table",
  * "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine",
  * "intra_apply", "intra_reeval", "intra_select", "commit", "unique", "zero";
- * learned search: "net_input", "net_forward", "propose_inter", "top_intra",
- * "lintra_pass") since it was enabled
+ * learned search: "net_input", "net_l12", "net_l3" (inference), "propose_inter",
+ * "top_intra", "lintra_pass"; "net_forward": training products) since it was enabled
  * (synchronises).  While enabled, steps run as individual launches, not graphs. */
 int32_t mptc_profile_enable(mptc_ctx* ctx, int32_t enable);
 int32_t mptc_profile_read(mptc_ctx* ctx, const char* name, int64_t* launches, double* total_ms);
@@ -177,9 +177,13 @@ int32_t mptc_profile_read(mptc_ctx* ctx, const char* name, int64_t* launches, do
  *
  * Parameters are one fp32 vector of MPTC_NET_PARAMS values: W1 [48][256],
  * b1 [256], W2 [256][256], b2 [256], W3 [256][262], b3 [262] (W[k][n]: input
- * k, output n).  Training loss: softmax cross-entropy of the logits against
- * the chosen offset + seed_weight * mean squared error of the seeds against
- * the re-fit endpoints / 255, averaged over the batch; Adam.
+ * k, output n).  Training loss: -log of the probability the softmax of the
+ * logits gives a target set of offsets (the offsets the exhaustive search
+ * found passing; one offset = cross-entropy) + seed_weight * mean squared
+ * error of the seeds against the endpoints / 255, averaged over the batch;
+ * Adam.
+ * Proposals: the top_k admitted offsets by score (scores compared as fp32
+ * truncated to 24 bits, ties to the lower logit index).
  * ------------------------------------------------------------------------- */
 #define MPTC_NET_INTER 0
 #define MPTC_NET_INTRA 1
@@ -214,29 +218,36 @@ int32_t mptc_net_get_params(mptc_ctx* ctx, int32_t net, float* params_out);
 int32_t mptc_net_set_params(mptc_ctx* ctx, int32_t net, const float* params);
 /* Forward pass over n blocks: d_px n x 48 bytes, d_out n x 262 fp32 (device). */
 int32_t mptc_net_forward(mptc_ctx* ctx, int32_t net, const uint8_t* d_px, int32_t n, float* d_out);
-/* Loss and its gradient over n pairs (device inputs: pixels n x 48, targets
- * n int32 logit indices, endpoints n x 6 bytes); grad_out: host,
- * MPTC_NET_PARAMS floats. */
-int32_t mptc_net_gradients(mptc_ctx* ctx, int32_t net, const uint8_t* d_px, const int32_t* d_target,
+/* Loss and its gradient over n pairs (device inputs: pixels n x 48 bytes,
+ * target sets n x 8 uint32 -- bit L of word L / 32 marks logit L as a
+ * target -- endpoints n x 6 bytes); grad_out: host, MPTC_NET_PARAMS floats. */
+int32_t mptc_net_gradients(mptc_ctx* ctx, int32_t net, const uint8_t* d_px, const uint32_t* d_target,
                            const uint8_t* d_eps, int32_t n, float seed_weight, float* grad_out,
                            float* loss_out);
 /* cfg->steps Adam steps on batches drawn without replacement (reshuffled
- * each epoch) from n_pairs device pairs; loss_out (host, optional) receives
- * each step's batch loss. */
-int32_t mptc_net_train(mptc_ctx* ctx, int32_t net, const uint8_t* d_px, const int32_t* d_target,
+ * each epoch) from n_pairs device pairs (layouts as mptc_net_gradients);
+ * loss_out (host, optional) receives each step's batch loss. */
+int32_t mptc_net_train(mptc_ctx* ctx, int32_t net, const uint8_t* d_px, const uint32_t* d_target,
                        const uint8_t* d_eps, int32_t n_pairs, const mptc_train_cfg* cfg,
                        float* loss_out);
-/* Training pairs from one encoded frame (device pointers: d_rgb, d_motion as
- * written by an encode with this search_area, d_eps its eps output): every
- * block reused from the window `net` scores appends (48 pixels, logit index,
- * 6 endpoint bytes) at *d_count (device counter, advanced; pairs beyond
+/* Training pairs from the exhaustive search.  With recording on, every
+ * exhaustive step also records, per block, which offsets of its inter and
+ * intra windows pass the accept test (inter: against the previous frame;
+ * intra: against the fixpoint words).  mptc_collect_pairs then appends, for
+ * network `net`, one pair per block of the last step whose set is not empty:
+ * 48 pixels to d_px, the set (8 words) to d_target, the block's endpoints
+ * after the encode (the frame's eps output, else its own fit's) to
+ * d_eps_out, at index *d_count (device counter, advanced; pairs beyond
  * capacity are dropped). */
-int32_t mptc_collect_pairs(mptc_ctx* ctx, int32_t net, const uint8_t* d_rgb, int32_t width,
-                           int32_t height, const uint8_t* d_motion, const uint8_t* d_eps,
-                           int32_t search_area, uint8_t* d_px, int32_t* d_target, uint8_t* d_eps_out,
-                           uint32_t* d_count, int32_t capacity);
+int32_t mptc_record_pairs(mptc_ctx* ctx, int32_t enable);
+int32_t mptc_collect_pairs(mptc_ctx* ctx, int32_t net, uint8_t* d_px, uint32_t* d_target,
+                           uint8_t* d_eps_out, uint32_t* d_count, int32_t capacity);
 /* Search mode and proposals per block (top_k 1..16) for later encodes. */
 int32_t mptc_set_search_mode(mptc_ctx* ctx, int32_t mode, int32_t top_k);
+/* The offsets (logit indices, -1 = none) the last learned step proposed:
+ * top_k per block of the step, raster order over its frames; out: host,
+ * n_blocks * top_k int16.  Blocks that did not search hold stale values. */
+int32_t mptc_get_proposals(mptc_ctx* ctx, int32_t net, int16_t* out, int64_t n_blocks);
 /* Replaces a network's scores by one fixed score per logit index (host,
  * MPTC_NET_LOGITS floats; NULL returns to the network): the same offsets are
  * proposed for every block. */

@@ -49,7 +49,8 @@ struct FrameDev {
   int32_t n_tiles;
   int32_t wtile_base;      // first wave tile of the search kernels
   int32_t tiles_x;         // wave tiles per row of wave tiles
-  int32_t pad[2];
+  int32_t ltile_base;      // first 8x8 tile of the learned intra passes
+  int32_t ltiles_x;        // 8x8 tiles per row
 };
 
 // Statistics counters are spread over kStatSlots copies (slot = workgroup
@@ -238,11 +239,16 @@ __global__ void __launch_bounds__(256) k_table_ref(const FrameDev* __restrict__ 
 // (r ascending), columns i = bx-S+t (t ascending), position r*2S + t.  Each
 // work item scans `rows` rows of the window and leaves a ChunkFold summary;
 // k_inter_combine folds them and applies the accept test (:890).
+//
+// With amask non-null, also records which offsets pass the accept test:
+// amask[g * 16 + dy + 8] bit dx + 8 for the offsets (dx, dy) of the 16x16
+// grid around the block (the training targets of the inter proposal network).
 __global__ void __launch_bounds__(256) k_inter(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ tsid,
                                                int n_wtiles, int tw, int n_chunks, int S,
                                                const uint64_t* __restrict__ own, const uint32_t* __restrict__ rw,
                                                const float4* __restrict__ rtab, uint4* __restrict__ summ,
-                                               size_t sstride, DevStats* __restrict__ st) {
+                                               size_t sstride, DevStats* __restrict__ st, int thr,
+                                               uint16_t* __restrict__ amask) {
   __shared__ uint8_t s_snap[512];
   __shared__ float2 s_ab[4];
   __shared__ unsigned long long s_acc;
@@ -262,16 +268,22 @@ __global__ void __launch_bounds__(256) k_inter(const FrameDev* __restrict__ fr, 
     const int t_lo = max(S - bx, 0), t_hi = min(bw - bx + S, W2);
     for (int r = chunk * rows; r < (chunk + 1) * rows; ++r) {
       const int j = by - S + r;
-      if (j < 0 || j >= bh) continue;
-      cf.begin_row();
-      for (int t = t_lo; t < t_hi; ++t) {
-        const int q = base + j * bw + (bx - S + t);
-        const float4 t4 = rtab[q];
-        const CandResult res = eval_candidate<false>(be, rw[q], CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap, s_ab);
-        cf.step(res.valid ? (int)sat16(res.err) : (int)kNoCand, r * W2 + t, r);
-        ++n_cand;
+      uint32_t rm = 0;
+      if (j >= 0 && j < bh) {
+        cf.begin_row();
+        for (int t = t_lo; t < t_hi; ++t) {
+          const int q = base + j * bw + (bx - S + t);
+          const float4 t4 = rtab[q];
+          const CandResult res = eval_candidate<false>(be, rw[q], CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap, s_ab);
+          cf.step(res.valid ? (int)sat16(res.err) : (int)kNoCand, r * W2 + t, r);
+          ++n_cand;
+          const int col = t - S + 8;
+          if (amask && res.valid && res.err <= thr && col >= 0 && col < 16) rm |= 1u << col;
+        }
+        cf.end_row(r);
       }
-      cf.end_row(r);
+      const int lrow = r - S + 8;
+      if (amask && lrow >= 0 && lrow < 16) amask[(size_t)g * 16 + lrow] = (uint16_t)rm;
     }
     summ[(size_t)chunk * sstride + g] = cf.pack();
   }
@@ -584,6 +596,32 @@ __global__ void __launch_bounds__(256) k_intra_select(const FrameDev* __restrict
     }
   }
   wg_append(changed, k, chg_list, &st->n_changed[par], &s_base);
+}
+
+// Which intra offsets pass the accept test at the fixpoint (the training
+// targets of the intra proposal network): amask[g * 16 + dy] bit dx + 8 for
+// the offsets (dx, dy = by - j) of the 16x16 causal grid, for every block
+// that searched the intra window.
+__global__ void __launch_bounds__(256) k_intra_amask(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
+                                                     int total, int S, int thr, const uint8_t* __restrict__ need,
+                                                     const uint2* __restrict__ E4, size_t nbs,
+                                                     uint16_t* __restrict__ amask) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total || !need[g]) return;
+  const int W2 = 2 * S;
+  for (int dy = 0; dy < 16; ++dy) {
+    uint32_t rm = 0;
+    if (dy < W2) {
+      for (int t = 0; t < W2; ++t) {
+        const int col = S - 1 - t + 8;
+        if (col < 0 || col >= 16) continue;
+        const int pos = dy * W2 + t;
+        const int e = e_at(E4[(size_t)(pos >> 2) * nbs + g], pos & 3);
+        if (e != kNoCand && e <= thr) rm |= 1u << col;
+      }
+    }
+    amask[(size_t)g * 16 + dy] = (uint16_t)rm;
+  }
 }
 
 // Commit (codec/dxt_image.cpp:900-905, 922-926, 953-954).

@@ -12,8 +12,9 @@
 //          (row dy = 0 holds only dx < 0: the window is causal)
 //
 // Training kernels: k_gemm_bf16, k_net_input, k_net_loss, k_colsum_part/final,
-// k_adam, k_to_bf16, k_collect_pairs.  Search kernels: k_gather_blocks,
-// k_propose_inter, k_top_intra, k_lintra_pass.
+// k_adam, k_to_bf16, k_collect_masks.  Search: k_frame_net_input (the
+// network input of a batch) and k_lintra_pass (learned intra fixpoint); the
+// inference kernels and the top-k proposal kernels are in net_infer.h.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -128,12 +129,15 @@ __global__ void __launch_bounds__(256) k_net_input(const uint8_t* __restrict__ p
   X[i] = f2bf((float)px[src * kNetIn + c] * (1.0f / 255.0f));
 }
 
-// Loss of a batch: softmax cross-entropy of the 256 logits against the chosen
-// offset, plus seed_w * the mean squared error of the 6 seeds against the
-// re-fit endpoints / 255.  Writes dY = d(mean loss)/dY in fp32 and bf16 and
-// adds the batch's mean loss to *loss.  One wave per row; target and eps rows
-// are taken through idx like the inputs.
-__global__ void __launch_bounds__(256) k_net_loss(const float* __restrict__ Y, int n, const int32_t* __restrict__ target,
+// Loss of a batch: -log of the probability the softmax of the 256 logits
+// gives the row's target set (tmask: 8 words, bit L of word L / 32; one bit =
+// the usual cross-entropy), plus seed_w * the mean squared error of the 6
+// seeds against the endpoints / 255.  d/dy_c of the set term is
+// p_c - [c in set] exp(y_c) / sum_{set} exp(y).  Writes dY = d(mean loss)/dY in
+// fp32 and bf16 and adds the batch's mean loss to *loss.  One wave per row;
+// target and eps rows are taken through idx like the inputs.  A row with an
+// empty set adds only its seed term.
+__global__ void __launch_bounds__(256) k_net_loss(const float* __restrict__ Y, int n, const uint32_t* __restrict__ tmask,
                                                   const uint8_t* __restrict__ eps, const int32_t* __restrict__ idx,
                                                   float seed_w, float* __restrict__ dYf, uint16_t* __restrict__ dYb,
                                                   float* __restrict__ loss) {
@@ -141,21 +145,34 @@ __global__ void __launch_bounds__(256) k_net_loss(const float* __restrict__ Y, i
   if (row >= n) return;
   const long src = idx ? (long)idx[row] : row;
   const float* y = Y + (size_t)row * kNetOut;
+  uint32_t tm[8];
+#pragma unroll
+  for (int w = 0; w < 8; ++w) tm[w] = tmask[src * 8 + w];
   float mx = -3.0e38f;
   for (int c = lane; c < kNetLogits; c += 64) mx = fmaxf(mx, y[c]);
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-  float se = 0.0f;
-  for (int c = lane; c < kNetLogits; c += 64) se += expf(y[c] - mx);
-  for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o);
-  const int tgt = target[src];
+  float se = 0.0f, ss = 0.0f;  // sum over all logits, over the set
+  for (int c = lane; c < kNetLogits; c += 64) {
+    const float e = expf(y[c] - mx);
+    se += e;
+    if ((tm[c >> 5] >> (c & 31)) & 1u) ss += e;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    se += __shfl_xor(se, o);
+    ss += __shfl_xor(ss, o);
+  }
   const float inv_n = 1.0f / (float)n, lse = logf(se);
-  float l = 0.0f;
-  for (int c = lane; c < kNetOut; c += 64) {
+  const bool has_set = ss > 0.0f;
+  float l = has_set ? (lane == 0 ? lse - logf(ss) : 0.0f) : 0.0f;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const int c = lane + 64 * j;
+    if (c >= kNetOut) break;
     float g;
     if (c < kNetLogits) {
-      const float p = expf(y[c] - mx - lse);
-      g = (p - (c == tgt ? 1.0f : 0.0f)) * inv_n;
-      if (c == tgt) l += lse - (y[c] - mx);
+      const float e = expf(y[c] - mx);
+      const float q = (has_set && ((tm[c >> 5] >> (c & 31)) & 1u)) ? e / ss : 0.0f;
+      g = has_set ? (e / se - q) * inv_n : 0.0f;
     } else {
       const float t = (float)eps[src * kNetSeeds + (c - kNetLogits)] * (1.0f / 255.0f);
       const float d = y[c] - t;
@@ -212,71 +229,67 @@ __global__ void __launch_bounds__(256) k_to_bf16(const float* __restrict__ x, in
   if (i < n) y[i] = f2bf(x[i]);
 }
 
-// Logit index of a decision's motion bytes (codec/dxt_image.cpp:760-764 for
-// inter, :683-687 for intra), or -1 when the offset lies outside the 16x16
-// grid (search areas above 8).
-__device__ __forceinline__ int logit_of_motion(int net, uint32_t mx, uint32_t my, int S) {
-  const int dx = (int)(mx & 0x7F) - S;
-  const int dy = net == 0 ? (int)(my & 0x7F) - S : 2 * S - 1 - (int)my;
-  if (dx < -8 || dx > 7) return -1;
-  if (net == 0) return (dy < -8 || dy > 7) ? -1 : (dy + 8) * kNetWin + (dx + 8);
-  return (dy < 0 || dy > 15) ? -1 : dy * kNetWin + (dx + 8);
-}
-
-// Training pairs of one encoded frame: every block the search reused from the
-// window this network scores (net 0: inter, motion x has bit 7; net 1:
-// intra) gives (its 48 pixels, the logit index of its offset, its re-fit
-// endpoints).  Pairs are appended in no particular order.
-__global__ void __launch_bounds__(256) k_collect_pairs(const uint8_t* __restrict__ rgb, int width, int bw, int nb,
-                                                       const uint8_t* __restrict__ motion,
-                                                       const uint8_t* __restrict__ eps, int S, int net,
-                                                       uint8_t* __restrict__ px_out, int32_t* __restrict__ tgt_out,
-                                                       uint8_t* __restrict__ eps_out, unsigned int* __restrict__ count,
-                                                       int capacity) {
+// Training pairs of the last encoded step: every block whose search window
+// (net 0: inter, blocks of inter frames; net 1: intra, blocks that searched
+// it) holds at least one offset that passes the accept test gives (its 48
+// pixels, the 256-bit set of passing offsets as 8 words, bit L of word L / 32,
+// its endpoints after the encode -- the frame's eps output, or its own fit's
+// when the frame has none).  amask: per block 16 rows of 16 offset bits, as
+// k_inter / k_intra_amask record them.  Pairs are appended in no particular
+// order; pairs beyond capacity are dropped (count still advances).
+__global__ void __launch_bounds__(256) k_collect_masks(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
+                                                       int total, int net, int S, const uint8_t* __restrict__ need,
+                                                       const uint16_t* __restrict__ amask,
+                                                       const uint64_t* __restrict__ own, uint8_t* __restrict__ px_out,
+                                                       uint32_t* __restrict__ tgt_out, uint8_t* __restrict__ eps_out,
+                                                       unsigned int* __restrict__ count, int capacity) {
   __shared__ unsigned int s_base;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int L = -1;
-  if (k < nb) {
-    const uint32_t mx = motion[2 * k], my = motion[2 * k + 1];
-    const bool reused = mx != 255u;
-    const bool inter = (mx & 0x80u) != 0;
-    if (reused && inter == (net == 0)) L = logit_of_motion(net, mx, my, S);
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool want = false;
+  if (g < total) {
+    const FrameDev& d = fr[sid[g]];
+    if (net == 0 ? !d.is_intra : need[g] != 0) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        // inter rows outside the window (search areas below 8) are not recorded
+        const bool row_ok = net == 1 || (r - 8 >= -S && r - 8 < S);
+        const uint32_t bits = row_ok ? (uint32_t)amask[(size_t)g * 16 + r] : 0u;
+        m[r >> 1] |= bits << (16 * (r & 1));
+        want = want || bits != 0;
+      }
+    }
   }
-  const bool want = L >= 0;
   if (threadIdx.x == 0) s_base = 0;
   __syncthreads();
-  const unsigned long long m = __ballot(want);
+  const unsigned long long bal = __ballot(want);
   const int lane = (int)__lane_id();
   unsigned int off = 0;
-  if (m && lane == 0) off = atomicAdd(&s_base, (unsigned int)__popcll(m));
+  if (bal && lane == 0) off = atomicAdd(&s_base, (unsigned int)__popcll(bal));
   off = __shfl(off, 0);
   __syncthreads();
   if (threadIdx.x == 0 && s_base) s_base = atomicAdd(count, s_base);
   __syncthreads();
   if (!want) return;
-  const unsigned slot = s_base + off + (unsigned)__popcll(m & ((1ull << lane) - 1));
+  const unsigned slot = s_base + off + (unsigned)__popcll(bal & ((1ull << lane) - 1));
   if (slot >= (unsigned)capacity) return;
-  BlockPx b;
-  load_block(rgb, width, k % bw, k / bw, b);
-  uint32_t* o = reinterpret_cast<uint32_t*>(px_out + (size_t)slot * kNetIn);
-  for (int w = 0; w < 12; ++w) o[w] = b.w[w];
-  tgt_out[slot] = L;
-  for (int c = 0; c < kNetSeeds; ++c) eps_out[(size_t)slot * kNetSeeds + c] = eps[(size_t)kNetSeeds * k + c];
-}
-
-// Block pixels of every block of a batch -> total x 48 bytes.
-__global__ void __launch_bounds__(256) k_gather_blocks(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
-                                                       int total, uint8_t* __restrict__ px) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
   const FrameDev& d = fr[sid[g]];
   const int k = g - d.base;
   BlockPx b;
   load_block(d.rgb, d.width, k % d.bw, k / d.bw, b);
-  uint4* o = reinterpret_cast<uint4*>(px + (size_t)g * kNetIn);
-  o[0] = make_uint4(b.w[0], b.w[1], b.w[2], b.w[3]);
-  o[1] = make_uint4(b.w[4], b.w[5], b.w[6], b.w[7]);
-  o[2] = make_uint4(b.w[8], b.w[9], b.w[10], b.w[11]);
+  uint32_t* o = reinterpret_cast<uint32_t*>(px_out + (size_t)slot * kNetIn);
+  for (int w = 0; w < 12; ++w) o[w] = b.w[w];
+  for (int w = 0; w < 8; ++w) tgt_out[(size_t)slot * 8 + w] = m[w];
+  uint8_t* e = eps_out + (size_t)slot * kNetSeeds;
+  if (d.eps) {
+    for (int c = 0; c < kNetSeeds; ++c) e[c] = d.eps[(size_t)kNetSeeds * k + c];
+  } else {
+    uint32_t r1, g1, b1, r2, g2, b2;
+    decode565((uint32_t)(own[g] & 0xFFFF), r1, g1, b1);
+    decode565((uint32_t)((own[g] >> 16) & 0xFFFF), r2, g2, b2);
+    e[0] = (uint8_t)r1; e[1] = (uint8_t)g1; e[2] = (uint8_t)b1;
+    e[3] = (uint8_t)r2; e[4] = (uint8_t)g2; e[5] = (uint8_t)b2;
+  }
 }
 
 // Network input of every block of a batch: total x 48 bf16 (pixel / 255).
@@ -303,135 +316,30 @@ __global__ void __launch_bounds__(256) k_frame_net_input(const FrameDev* __restr
   }
 }
 
-// Top-k logit indices (descending score, ties to the lower index) among the
-// offsets `valid` admits.  Scores are Ys[L * ys_l + g * ys_g]: the logits of
-// the network (transposed, ys_l = rows) or one shared prior (ys_g = 0).
-template <typename Valid>
-__device__ __forceinline__ void top_k(const float* __restrict__ Ys, long ys_l, long ys_g, int g, int kk,
-                                      Valid&& valid, int (&ti)[kMaxTopK]) {
-  float tv[kMaxTopK];
-#pragma unroll
-  for (int i = 0; i < kMaxTopK; ++i) {
-    tv[i] = -3.0e38f;
-    ti[i] = -1;
-  }
-  const float* y = Ys + (long)g * ys_g;
-  for (int L = 0; L < kNetLogits; ++L) {
-    if (!valid(L)) continue;
-    float v = y[(long)L * ys_l];
-    if (v != v) v = -3.0e38f;
-    int cand = L;
-    // insertion into the sorted list (static indices keep it in registers)
-#pragma unroll
-    for (int i = 0; i < kMaxTopK; ++i) {
-      if (i < kk && cand >= 0 && (ti[i] < 0 || v > tv[i])) {
-        const float t = tv[i];
-        const int u = ti[i];
-        tv[i] = v;
-        ti[i] = cand;
-        v = t;
-        cand = u;
-      }
-    }
-  }
-}
+// Learned intra search, one fixpoint pass.  A wave owns an 8x8 tile of
+// blocks (lane = 8 * row + column in the tile).  Every block that needs the
+// intra search evaluates its proposed offsets against the current words --
+// words of its own tile come from the lanes that own them, others from
+// memory -- and once all its evaluations are current it takes the valid
+// proposal with the smallest error delta (first in rank order on ties),
+// accepts it if the delta passes the threshold, and its resulting word
+// becomes current.  Each round a lane redoes one evaluation whose word
+// changed (seen/errs keep the others).  Proposals only point at earlier blocks
+// in raster order, which inside a tile are lower lanes, so the rounds settle
+// the tile's chains within the pass.  Words from other tiles are read once per
+// pass; a change there is caught by the next pass.  Each block's window holds
+// only earlier blocks, so a pass that changes no word ends at the unique
+// fixpoint.  A tile not settled after kMaxRounds rounds stores, for every
+// lane, the decision its current evaluations give; what is stale is redone in
+// the next pass (never-done evaluations are stored as kNotEvaluated).  Pass
+// counters: this pass adds its changed words to n_changed[par] and clears
+// n_changed[par ^ 1] for the next pass.
+constexpr int kMaxRounds = 512;
+constexpr int kNotEvaluated = (int)0x80000000;
 
-// Learned inter search: the top_k offsets are evaluated exactly (the same
-// re-fit and error as InterBlockSearch, codec/dxt_image.cpp:715-774); the
-// valid one with the smallest error delta (first in score order on ties) is
-// reused if the delta passes the threshold (:890).
-__global__ void __launch_bounds__(256) k_propose_inter(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
-                                                       int total, int S, int thr, int kk,
-                                                       const float* __restrict__ Ys, long ys_l, long ys_g,
-                                                       const uint64_t* __restrict__ own,
-                                                       const uint32_t* __restrict__ rw,
-                                                       const float4* __restrict__ rtab,
-                                                       uint64_t* __restrict__ dec_out, DevStats* __restrict__ st) {
-  __shared__ uint8_t s_snap[512];
-  __shared__ float2 s_ab[4];
-  __shared__ unsigned long long s_acc;
-  load_lds_tables(s_snap, s_ab);
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long n_cand = 0;
-  if (g < total && !fr[sid[g]].is_intra) {
-    const FrameDev& d = fr[sid[g]];
-    const int k = g - d.base, bx = k % d.bw, by = k / d.bw, bw = d.bw, bh = d.bh;
-    const int lo = -min(S, 8), hi = min(S, 8) - 1;
-    auto valid = [&](int L) {
-      const int dy = L / kNetWin - 8, dx = L % kNetWin - 8;
-      return dy >= lo && dy <= hi && dx >= lo && dx <= hi && by + dy >= 0 && by + dy < bh && bx + dx >= 0 &&
-             bx + dx < bw;
-    };
-    int ti[kMaxTopK];
-    top_k(Ys, ys_l, ys_g, g, kk, valid, ti);
-    BlockPx px;
-    load_block(d.rgb, d.width, bx, by, px);
-    BlockEval be;
-    make_block_eval(px, own[g], be);
-    int best = 0x7fffffff, best_L = -1;
-    uint32_t best_c = 0;
-#pragma unroll
-    for (int i = 0; i < kMaxTopK; ++i) {
-      const int L = i < kk ? ti[i] : -1;
-      if (L >= 0) {
-        const int q = d.base + (by + L / kNetWin - 8) * bw + (bx + L % kNetWin - 8);
-        const float4 t4 = rtab[q];
-        const uint32_t c = rw[q];
-        const CandResult res = eval_candidate<false>(be, c, CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap, s_ab);
-        ++n_cand;
-        if (res.valid && res.err < best) {
-          best = res.err;
-          best_L = L;
-          best_c = c;
-        }
-      }
-    }
-    uint64_t dc = 0;
-    if (best_L >= 0 && best <= thr)
-      dc = pack_dec(best_c, (uint32_t)(best_L % kNetWin - 8 + S) | 0x80u, (uint32_t)(best_L / kNetWin - 8 + S) | 0x80u,
-                    true);
-    dec_out[g] = dc;
-  }
-  wg_count(&s_acc, n_cand, st->inter_cands);
-}
-
-// Learned intra search, step 1: the top_k causal-window offsets of every
-// block that needs the intra search (prop[g * kk + i], -1 past the end).
-__global__ void __launch_bounds__(256) k_top_intra(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
-                                                   int total, int S, int kk, const float* __restrict__ Ys, long ys_l,
-                                                   long ys_g, const uint8_t* __restrict__ need,
-                                                   int16_t* __restrict__ prop) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total || !need[g]) return;
-  const FrameDev& d = fr[sid[g]];
-  const int k = g - d.base, bx = k % d.bw, by = k / d.bw, bw = d.bw;
-  const int lo = -min(S, 8), hi = min(S, 8) - 1, dmax = min(2 * S - 1, 15);
-  auto valid = [&](int L) {
-    const int dy = L / kNetWin, dx = L % kNetWin - 8;
-    return dy <= dmax && dx >= lo && dx <= hi && by - dy >= 0 && bx + dx >= 0 && bx + dx < bw && (dy > 0 || dx < 0);
-  };
-  int ti[kMaxTopK];
-  top_k(Ys, ys_l, ys_g, g, kk, valid, ti);
-#pragma unroll
-  for (int i = 0; i < kMaxTopK; ++i)
-    if (i < kk) prop[(size_t)g * kk + i] = (int16_t)ti[i];
-}
-
-// Learned intra search, one fixpoint pass.  Every block that needs the intra
-// search compares the current words at its proposed offsets with the words it
-// last evaluated; where one differs (or on the first pass) it re-evaluates the
-// changed ones, takes the valid proposal with the smallest error delta (first
-// in score order on ties), accepts it if the delta passes the threshold, and
-// writes its resulting word as current at once.  The candidate table of a
-// word is derived from the word itself, so a word read while another
-// workgroup rewrites it is still evaluated consistently; a stale read is
-// caught by the next pass's comparison.  Each block's window holds only
-// earlier blocks, so a pass that changes no word ends at the unique fixpoint.
-// Pass counters: this pass adds its changed words to n_changed[par] and
-// clears n_changed[par ^ 1] for the next pass.
-__global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
-                                                     int total, int S, int thr, int kk,
-                                                     const int16_t* __restrict__ prop,
+template <int K>
+__global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict__ fr, int n_frames, int n_ltiles,
+                                                     int S, int thr, int kk, const int16_t* __restrict__ prop,
                                                      const uint64_t* __restrict__ own, const uint8_t* __restrict__ need,
                                                      uint32_t* __restrict__ cw, uint32_t* __restrict__ seen,
                                                      int* __restrict__ errs, uint64_t* __restrict__ dec,
@@ -444,64 +352,127 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
   if (blockIdx.x == 0 && threadIdx.x == 0) st->n_changed[par ^ 1] = 0;
   if (threadIdx.x == 0) s_chg = 0;
   __syncthreads();
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long n_cand = 0;
-  if (g < total && need[g]) {
-    const FrameDev& d = fr[sid[g]];
-    const int k = g - d.base, bx = k % d.bw, by = k / d.bw;
-    uint32_t w[kMaxTopK];
-    bool dirty = first != 0;
+  const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const int lane = (int)(threadIdx.x & 63);
+  unsigned long long n_cand = 0, n_re = 0;  // first evaluations, re-evaluations
+  if (wv < n_ltiles) {
+    int f = 0;
+    while (f + 1 < n_frames && fr[f + 1].ltile_base <= wv) ++f;
+    const FrameDev& d = fr[f];
+    const int lt = wv - d.ltile_base, tx = lt % d.ltiles_x, ty = lt / d.ltiles_x;
+    const int bx = tx * 8 + (lane & 7), by = ty * 8 + (lane >> 3);
+    const bool inside = bx < d.bw && by < d.bh;
+    const int g = inside ? d.base + by * d.bw + bx : 0;
+    const bool act = inside && need[g];
+    uint32_t cur = inside ? cw[g] : 0u;
+    const uint32_t start = cur;
+    int src[K];  // lane holding the proposal's block, or -1 (other tile / none)
+    uint32_t w[K], sn[K];
+    int er[K], Ls[K];
 #pragma unroll
-    for (int i = 0; i < kMaxTopK; ++i) {
+    for (int i = 0; i < K; ++i) {
+      const int L = (act && i < kk) ? (int)prop[(size_t)g * kk + i] : -1;
+      Ls[i] = L;
+      src[i] = -1;
       w[i] = 0;
-      if (i < kk) {
-        const int L = prop[(size_t)g * kk + i];
-        if (L >= 0) {
-          const int q = d.base + (by - L / kNetWin) * d.bw + (bx + L % kNetWin - 8);
-          w[i] = __builtin_nontemporal_load(&cw[q]);
-          if (!first && w[i] != seen[(size_t)g * kk + i]) dirty = true;
+      sn[i] = 0;
+      er[i] = kNotEvaluated;
+      if (L >= 0) {
+        const int qx = bx + L % kNetWin - 8, qy = by - L / kNetWin;
+        if ((qx >> 3) == tx && (qy >> 3) == ty)
+          src[i] = (qy & 7) * 8 + (qx & 7);
+        else
+          w[i] = cw[d.base + qy * d.bw + qx];
+        if (!first) {
+          sn[i] = seen[(size_t)g * kk + i];
+          er[i] = errs[(size_t)g * kk + i];
         }
       }
     }
-    if (dirty) {
-      BlockPx px;
-      load_block(d.rgb, d.width, bx, by, px);
-      BlockEval be;
-      const uint64_t o = own[g];
-      make_block_eval(px, o, be);
-      int best = 0x7fffffff, bi = -1, bL = -1;
+    BlockEval be;
+    bool have_be = false, published = false, clean_once = first == 0;
+    uint64_t dc = 0;
+    // the decision the current evaluations give: dc, and the word it makes current
+    auto decide = [&]() {
+      int best = 0x7fffffff, bi = -1;
 #pragma unroll
-      for (int i = 0; i < kMaxTopK; ++i) {
-        const int L = i < kk ? (int)prop[(size_t)g * kk + i] : -1;
-        if (L >= 0) {
-          const size_t s = (size_t)g * kk + i;
-          int e;
-          if (first || w[i] != seen[s]) {
-            const CandResult res = eval_candidate<false>(be, w[i], cand_table(w[i]), s_snap, s_ab);
-            ++n_cand;
-            e = res.valid ? res.err : 0x7fffffff;
-            seen[s] = w[i];
-            errs[s] = e;
-          } else {
-            e = errs[s];
-          }
-          if (e < best) {
-            best = e;
-            bi = i;
-            bL = L;
-          }
+      for (int i = 0; i < K; ++i)
+        if (Ls[i] >= 0 && er[i] != kNotEvaluated && er[i] < best) {
+          best = er[i];
+          bi = i;
         }
+      dc = 0;
+      uint32_t nw = (uint32_t)(own[g] >> 32);
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        if (i == bi && best <= thr) {
+          const int dy = Ls[i] / kNetWin, dx = Ls[i] % kNetWin - 8;
+          dc = pack_dec(sn[i], (uint32_t)(dx + S), (uint32_t)(2 * S - 1 - dy), true);
+          nw = sn[i];
+        }
+      cur = nw;
+      published = true;
+      clean_once = true;
+    };
+    for (int round = 0; round < kMaxRounds; ++round) {
+      // current words of the proposals inside the tile
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const uint32_t v = __shfl(cur, src[i] >= 0 ? src[i] : lane);
+        if (src[i] >= 0) w[i] = v;
       }
-      uint64_t dc = 0;
-      uint32_t nw = (uint32_t)(o >> 32);
-      if (bi >= 0 && best <= thr) {
-        const int dy = bL / kNetWin, dx = bL % kNetWin - 8;
-        dc = pack_dec(w[bi], (uint32_t)(dx + S), (uint32_t)(2 * S - 1 - dy), true);
-        nw = w[bi];
+      int j = -1;  // first proposal whose evaluation is stale
+#pragma unroll
+      for (int i = K - 1; i >= 0; --i)
+        if (Ls[i] >= 0 && (er[i] == kNotEvaluated || w[i] != sn[i])) j = i;
+      const bool dirty = act && j >= 0;
+      // a lane publishes once nothing of it is stale (first pass: after its
+      // first full set of evaluations even if nothing changed)
+      const bool publish = act && j < 0 && !clean_once;
+      if (__ballot(dirty || publish) == 0ull) break;
+      if (dirty) {
+        if (!have_be) {
+          BlockPx px;
+          load_block(d.rgb, d.width, bx, by, px);
+          make_block_eval(px, own[g], be);
+          have_be = true;
+        }
+        uint32_t wj = 0;
+        bool fresh = false;
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+          if (i == j) {
+            wj = w[i];
+            fresh = er[i] == kNotEvaluated;
+          }
+        const CandResult res = eval_candidate<false>(be, wj, cand_table(wj), s_snap, s_ab);
+        const int e = res.valid ? res.err : 0x7fffffff;
+        if (fresh)
+          ++n_cand;
+        else
+          ++n_re;
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+          if (i == j) {
+            sn[i] = wj;
+            er[i] = e;
+          }
+        clean_once = false;  // something was re-evaluated: decide again when clean
+      } else if (publish) {
+        decide();
       }
+    }
+    if (act && !clean_once) decide();  // round limit reached: store a consistent state
+    if (published) {
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        if (Ls[i] >= 0) {
+          seen[(size_t)g * kk + i] = sn[i];
+          errs[(size_t)g * kk + i] = er[i];
+        }
       dec[g] = dc;
-      if (nw != cw[g]) {
-        cw[g] = nw;
+      if (cur != start) {
+        cw[g] = cur;
         atomicAdd(&s_chg, 1u);
       }
     }
@@ -511,7 +482,8 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
     const unsigned old = atomicAdd(&st->n_changed[par], s_chg);
     if (old == 0) atomicAdd(&st->active_passes, 1u);
   }
-  wg_count(&s_acc, n_cand, first ? st->intra_cands : st->reeval_cands);
+  wg_count(&s_acc, n_cand, st->intra_cands);
+  wg_count(&s_acc, n_re, st->reeval_cands);
 }
 
 }  // namespace mptc

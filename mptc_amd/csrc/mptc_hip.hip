@@ -24,7 +24,7 @@
 #include <vector>
 
 #include "../../include/mptc.h"
-#include "mlp_kernels.h"
+#include "net_infer.h"
 #include "framegen.h"
 
 using namespace mptc;
@@ -102,13 +102,14 @@ int init_tables(int device) {
 
 enum KernelId {
   KF_ZERO, KF_FIT, KF_TABLE, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
-  KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_N
+  KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
+  KF_NET_L3, KF_N
 };
 const char* kKernelNames[KF_N] = {"zero",         "fit",          "table",       "inter",
                                   "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
                                   "intra_apply",  "intra_reeval", "intra_select", "commit",
                                   "unique",       "net_input",   "net_forward", "propose_inter",
-                                  "top_intra",    "lintra_pass"};
+                                  "top_intra",    "lintra_pass",  "net_l12",     "net_l3"};
 constexpr int kMaxChunks = 8;  // window-row chunks per block in the search kernels
 
 constexpr int kReevalGrid = 4096;  // workgroups of the re-evaluation pass (grid-stride)
@@ -134,6 +135,7 @@ struct NetState {
   bool ready = false;      // parameters set
   float* prior = nullptr;  // kNetLogits fixed scores
   bool use_prior = false;
+  uint16_t* pk = nullptr;  // packed inference image (net_infer.h)
 };
 
 // Activations of a forward (and backward) pass over `rows` blocks.
@@ -183,7 +185,7 @@ struct mptc_ctx {
   // batch shape the maps and graphs were built for
   std::vector<int> shape;
   mptc_tuning shape_tune{0, 0, 0, 0, 0};
-  int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0;
+  int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0, n_ltiles = 0;
   Graphs graphs;
   int last_passes = 0, last_total = 0, last_intra = 0;
   // learned proposals
@@ -191,8 +193,13 @@ struct mptc_ctx {
   NetState net[2];
   NetWork nw;  // training and standalone forward passes
   NetWork ew;  // the encode step's forward passes (max_total rows)
-  float* d_yt = nullptr;     // the step's logits, transposed: kNetOut x max_total
-  int16_t* d_prop = nullptr; // intra proposals, kMaxTopK per block
+  int n_cu = 256;             // compute units of the device
+  uint4* d_h2p = nullptr;     // layer-2 activations of the step (net_infer.h tiles)
+  int16_t* d_pinter = nullptr;  // inter proposals, kMaxTopK per block
+  int16_t* d_prop = nullptr;    // intra proposals, kMaxTopK per block
+  bool record = false;          // exhaustive steps record acceptance masks
+  bool last_recorded = false;   // the last step did
+  uint16_t* d_amask[2] = {nullptr, nullptr};  // inter, intra: 16 rows of 16 offset bits per block
   uint32_t* d_seen = nullptr;
   int* d_errs = nullptr;
   // profiling
@@ -284,13 +291,14 @@ int net_alloc(NetState& n) {
   if (n.w) return MPTC_OK;
   int rc;
   if ((rc = dalloc(&n.w, kNetParams)) || (rc = dalloc(&n.wb, kNetParams)) || (rc = dalloc(&n.m, kNetParams)) ||
-      (rc = dalloc(&n.v, kNetParams)) || (rc = dalloc(&n.g, kNetParams)) || (rc = dalloc(&n.prior, kNetLogits)))
+      (rc = dalloc(&n.v, kNetParams)) || (rc = dalloc(&n.g, kNetParams)) || (rc = dalloc(&n.prior, kNetLogits)) ||
+      (rc = dalloc(&n.pk, kPkSize)))
     return rc;
   return MPTC_OK;
 }
 
 void net_free(NetState& n) {
-  void* b[] = {n.w, n.wb, n.m, n.v, n.g, n.prior};
+  void* b[] = {n.w, n.wb, n.m, n.v, n.g, n.prior, n.pk};
   for (void* p : b)
     if (p) (void)hipFree(p);
   n = NetState();
@@ -375,7 +383,7 @@ int colsum_q(mptc_ctx* c, hipStream_t s, const float* X, int rows, int cols, lon
 
 // Loss and gradient of a forward pass over w (rows = the batch): n.g receives
 // d(mean loss)/d(parameters), *loss (device) the mean loss.
-int net_backward_q(mptc_ctx* c, hipStream_t s, NetState& n, NetWork& w, int rows, const int32_t* target,
+int net_backward_q(mptc_ctx* c, hipStream_t s, NetState& n, NetWork& w, int rows, const uint32_t* target,
                    const uint8_t* eps, const int32_t* idx, float seed_w, float* loss) {
   int rc;
   if ((rc = launch(c, s, KF_NET_FWD, [&] {
@@ -447,7 +455,7 @@ int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
   const int tw = c->tune.tile_w, th = 64 / tw;
   std::vector<int> shape;
   FrameDev* h = c->h_fr + (size_t)c->fr_slot * kMaxFrames;
-  int base = 0, tb = 0, wt = 0;
+  int base = 0, tb = 0, wt = 0, lt = 0;
   for (int f = 0; f < n; ++f) {
     const FrameIn& x = in[f];
     if (x.w <= 0 || x.h <= 0 || (x.w & 3) || (x.h & 3))
@@ -462,6 +470,9 @@ int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
     d.tile_base = tb; d.n_tiles = (d.nb + kUniqueTile - 1) / kUniqueTile;
     d.tiles_x = (d.bw + tw - 1) / tw;
     d.wtile_base = wt;
+    d.ltile_base = lt;
+    d.ltiles_x = (d.bw + 7) / 8;
+    lt += d.ltiles_x * ((d.bh + 7) / 8);
     base += d.nb;
     tb += d.n_tiles;
     wt += d.tiles_x * ((d.bh + th - 1) / th);
@@ -498,29 +509,57 @@ int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
   c->total = base;
   c->n_wtiles = wt;
   c->n_tiles = tb;
+  c->n_ltiles = lt;
   return MPTC_OK;
 }
 
-// Scores of network `net` for every block of the batch into d_yt (nothing
-// when a fixed prior replaces the network).
-int learned_scores_q(mptc_ctx* c, hipStream_t s, int net) {
-  const NetState& n = c->net[net];
-  if (n.use_prior) return MPTC_OK;
-  const int total = c->total;
-  int rc = launch(c, s, KF_NET_INPUT, [&] {
-    hipLaunchKernelGGL(k_frame_net_input, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total,
-                       c->ew.X);
+// Fused inference over w.X -> H2 tiles (d_h2p) -> either Y (row-major) or
+// the top-kk admitted offsets of each block into prop.
+int infer_q(mptc_ctx* c, hipStream_t s, const NetState& n, const uint16_t* X, int rows, uint4* h2p, float* Y,
+            int net, int kk, const uint8_t* need, int16_t* prop) {
+  const int n_tiles = (rows + 31) / 32;
+  const unsigned grid = (unsigned)std::max(1, std::min(c->n_cu, (n_tiles + kInferWaves - 1) / kInferWaves));
+  const int S = c->cfg.search_area;
+  int rc = launch(c, s, KF_NET_L12, [&] {
+    hipLaunchKernelGGL(k_net_l12, dim3(grid), dim3(64 * kInferWaves), 0, s, X, rows, n.pk, h2p);
   });
   if (rc) return rc;
-  return net_forward_q(c, s, n, c->ew, total, c->d_yt, true);
+  return launch(c, s, KF_NET_L3, [&] {
+    auto kern = kk <= 4 ? k_net_l3<4> : kk <= 8 ? k_net_l3<8> : k_net_l3<16>;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kInferWaves), 0, s, h2p, rows, n.pk, Y, c->d_fr, c->d_sid, net,
+                       S, kk, need, prop);
+  });
+}
+
+// Proposals of network `net` for the batch's blocks (need: the blocks that
+// search; null = the blocks of inter frames) into prop.  The network input
+// is built once per step (x_done).
+int proposals_q(mptc_ctx* c, hipStream_t s, int net, const uint8_t* need, int16_t* prop, bool& x_done) {
+  const NetState& n = c->net[net];
+  const int total = c->total;
+  if (n.use_prior)
+    return launch(c, s, KF_TOP_INTRA, [&] {
+      hipLaunchKernelGGL(k_top_scores, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total, net,
+                         c->cfg.search_area, c->top_k, n.prior, 1L, 0L, need, prop);
+    });
+  if (!x_done) {
+    int rc = launch(c, s, KF_NET_INPUT, [&] {
+      hipLaunchKernelGGL(k_frame_net_input, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total,
+                         c->ew.X);
+    });
+    if (rc) return rc;
+    x_done = true;
+  }
+  return infer_q(c, s, n, c->ew.X, total, c->d_h2p, nullptr, net, c->top_k, need, prop);
 }
 
 int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first) {
-  const int total = c->total, S = c->cfg.search_area, thr = c->cfg.err_threshold;
+  const int S = c->cfg.search_area, thr = c->cfg.err_threshold, kk = c->top_k;
   return launch(c, s, KF_LPASS, [&] {
-    hipLaunchKernelGGL(k_lintra_pass, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total, S, thr,
-                       c->top_k, c->d_prop, c->d_own, c->d_need, c->d_cw, c->d_seen, c->d_errs, c->d_intra,
-                       c->d_stats, par, first);
+    auto kern = kk <= 4 ? k_lintra_pass<4> : kk <= 8 ? k_lintra_pass<8> : k_lintra_pass<16>;
+    hipLaunchKernelGGL(kern, dim3(grid_for(c->n_ltiles * 64, 256)), dim3(256), 0, s, c->d_fr, c->n_frames,
+                       c->n_ltiles, S, thr, kk, c->d_prop, c->d_own, c->d_need, c->d_cw, c->d_seen, c->d_errs,
+                       c->d_intra, c->d_stats, par, first);
   });
 }
 
@@ -529,6 +568,7 @@ int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first) {
 int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
   const int total = c->total, wg = c->tune.wg_size, tw = c->tune.tile_w;
   const int S = c->cfg.search_area, thr = c->cfg.err_threshold;
+  bool x_done = false;
   int rc;
   if ((rc = launch(c, s, KF_ZERO, [&] {
          hipLaunchKernelGGL(k_zero_u32, dim3(grid_for((int)(sizeof(DevStats) / 4), 256)), dim3(256), 0, s,
@@ -551,7 +591,8 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
   if (c->mode == MPTC_SEARCH_EXHAUSTIVE) {
     if ((rc = launch(c, s, KF_INTER, [&] {
            hipLaunchKernelGGL(k_inter, dim3(gwk), dim3(wg), 0, s, c->d_fr, c->d_tsid, c->n_wtiles, tw, K, S,
-                              c->d_own, c->d_rw, c->d_rtab, c->d_summ, c->nbs, c->d_stats);
+                              c->d_own, c->d_rw, c->d_rtab, c->d_summ, c->nbs, c->d_stats, thr,
+                              c->record ? c->d_amask[0] : nullptr);
          })))
       return rc;
     if ((rc = launch(c, s, KF_INTER_COMBINE, [&] {
@@ -560,13 +601,10 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
          })))
       return rc;
   } else {
-    if ((rc = learned_scores_q(c, s, MPTC_NET_INTER))) return rc;
-    const float* ys = c->net[0].use_prior ? c->net[0].prior : c->d_yt;
-    const long ys_l = c->net[0].use_prior ? 1 : total, ys_g = c->net[0].use_prior ? 0 : 1;
+    if ((rc = proposals_q(c, s, MPTC_NET_INTER, nullptr, c->d_pinter, x_done))) return rc;
     if ((rc = launch(c, s, KF_PROPOSE, [&] {
-           hipLaunchKernelGGL(k_propose_inter, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
-                              S, thr, c->top_k, ys, ys_l, ys_g, c->d_own, c->d_rw, c->d_rtab, c->d_inter,
-                              c->d_stats);
+           hipLaunchKernelGGL(k_eval_inter, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total, S,
+                              thr, c->top_k, c->d_pinter, c->d_own, c->d_rw, c->d_rtab, c->d_inter, c->d_stats);
          })))
       return rc;
   }
@@ -576,14 +614,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
        })))
     return rc;
   if (c->mode == MPTC_SEARCH_LEARNED) {
-    if ((rc = learned_scores_q(c, s, MPTC_NET_INTRA))) return rc;
-    const float* ys = c->net[1].use_prior ? c->net[1].prior : c->d_yt;
-    const long ys_l = c->net[1].use_prior ? 1 : total, ys_g = c->net[1].use_prior ? 0 : 1;
-    if ((rc = launch(c, s, KF_TOP_INTRA, [&] {
-           hipLaunchKernelGGL(k_top_intra, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total, S,
-                              c->top_k, ys, ys_l, ys_g, c->d_need, c->d_prop);
-         })))
-      return rc;
+    if ((rc = proposals_q(c, s, MPTC_NET_INTRA, c->d_need, c->d_prop, x_done))) return rc;
     return lintra_pass_q(c, s, 0, 1);
   }
   if ((rc = launch(c, s, KF_DENSE, [&] {
@@ -639,6 +670,12 @@ int queue_passes(mptc_ctx* c, hipStream_t s) {
 int queue_suffix(mptc_ctx* c, hipStream_t s) {
   const int total = c->total, wg = c->tune.wg_size;
   int rc;
+  if (c->record && c->mode == MPTC_SEARCH_EXHAUSTIVE &&
+      (rc = launch(c, s, KF_COMMIT, [&] {
+         hipLaunchKernelGGL(k_intra_amask, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total,
+                            c->cfg.search_area, c->cfg.err_threshold, c->d_need, c->d_E, c->nbs, c->d_amask[1]);
+       })))
+    return rc;
   if ((rc = launch(c, s, KF_COMMIT, [&] {
          hipLaunchKernelGGL(k_commit, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total, c->d_own,
                             c->d_inter, c->d_need, c->d_intra, c->d_tiles, c->d_stats);
@@ -717,6 +754,7 @@ int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
   }
   c->last_passes = passes;
   c->last_total = c->total;
+  c->last_recorded = c->record && c->mode == MPTC_SEARCH_EXHAUSTIVE;
   return MPTC_OK;
 }
 
@@ -760,6 +798,11 @@ int32_t mptc_create(const mptc_config* cfg, mptc_ctx** out) {
   if ((rc = init_tables(cfg->device))) return rc;
   mptc_ctx* c = new mptc_ctx();
   c->cfg = *cfg;
+  {
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, cfg->device) == hipSuccess && cu > 0)
+      c->n_cu = cu;
+  }
   c->alloc_S = cfg->search_area;
   c->max_frames = frames;
   c->max_total = (int)total;
@@ -812,7 +855,7 @@ void mptc_destroy(mptc_ctx* c) {
   net_free(c->net[1]);
   c->nw.release();
   c->ew.release();
-  void* lbufs[] = {c->d_yt, c->d_prop, c->d_seen, c->d_errs};
+  void* lbufs[] = {c->d_h2p, c->d_pinter, c->d_prop, c->d_seen, c->d_errs, c->d_amask[0], c->d_amask[1]};
   for (void* p : lbufs)
     if (p) (void)hipFree(p);
   if (c->h_stats) (void)hipHostFree(c->h_stats);
@@ -1033,6 +1076,8 @@ static int upload_params(mptc_ctx* c, NetState& n, const float* params) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemsetAsync(n.m, 0, sizeof(float) * kNetParams, s));
   HIP_TRY(hipMemsetAsync(n.v, 0, sizeof(float) * kNetParams, s));
+  hipLaunchKernelGGL(k_pack_infer, dim3(grid_for(kL3Rows * kNetHid, 256)), dim3(256), 0, s, n.w, n.pk);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s));
   n.t = 0;
   n.ready = true;
@@ -1077,14 +1122,21 @@ int32_t mptc_net_forward(mptc_ctx* c, int32_t net, const uint8_t* d_px, int32_t 
   if (!d_px || !d_out || n < 1) return fail(MPTC_EINVAL, "bad argument");
   if (!c->net[net].ready) return fail(MPTC_ESTATE, "network parameters not set");
   hipStream_t s = c->stream;
+  uint4* h2p = nullptr;
+  if ((rc = dalloc(&h2p, (size_t)(n + 31) / 32 * (kH2TileBytes / 16)))) return rc;
   if ((rc = work_ensure(c->nw, n, false)) || (rc = net_input_q(c, s, d_px, nullptr, n, c->nw.X)) ||
-      (rc = net_forward_q(c, s, c->net[net], c->nw, n, d_out, false)))
+      (rc = infer_q(c, s, c->net[net], c->nw.X, n, h2p, d_out, net, 1, nullptr, nullptr))) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(h2p);
     return rc;
-  HIP_TRY(hipStreamSynchronize(s));
+  }
+  const hipError_t e = hipStreamSynchronize(s);
+  (void)hipFree(h2p);
+  HIP_TRY(e);
   return MPTC_OK;
 }
 
-int32_t mptc_net_gradients(mptc_ctx* c, int32_t net, const uint8_t* d_px, const int32_t* d_target,
+int32_t mptc_net_gradients(mptc_ctx* c, int32_t net, const uint8_t* d_px, const uint32_t* d_target,
                            const uint8_t* d_eps, int32_t n, float seed_w, float* grad_out, float* loss_out) {
   int rc = check_net(c, net);
   if (rc) return rc;
@@ -1113,7 +1165,7 @@ int32_t mptc_net_gradients(mptc_ctx* c, int32_t net, const uint8_t* d_px, const 
   return MPTC_OK;
 }
 
-int32_t mptc_net_train(mptc_ctx* c, int32_t net, const uint8_t* d_px, const int32_t* d_target, const uint8_t* d_eps,
+int32_t mptc_net_train(mptc_ctx* c, int32_t net, const uint8_t* d_px, const uint32_t* d_target, const uint8_t* d_eps,
                        int32_t n_pairs, const mptc_train_cfg* cfg, float* loss_out) {
   int rc = check_net(c, net);
   if (rc) return rc;
@@ -1169,26 +1221,39 @@ int32_t mptc_net_train(mptc_ctx* c, int32_t net, const uint8_t* d_px, const int3
                        kNetParams, lr, b1, b2, c1, c2, epsv);
     if (hipGetLastError() != hipSuccess) return done(fail(MPTC_EHIP, "k_adam launch failed"));
   }
+  hipLaunchKernelGGL(k_pack_infer, dim3(grid_for(kL3Rows * kNetHid, 256)), dim3(256), 0, s, ns.w, ns.pk);
+  if (hipGetLastError() != hipSuccess) return done(fail(MPTC_EHIP, "k_pack_infer launch failed"));
   if (loss_out &&
       hipMemcpyAsync(loss_out, d_losses, sizeof(float) * cfg->steps, hipMemcpyDeviceToHost, s) != hipSuccess)
     return done(fail(MPTC_EHIP, "hipMemcpyAsync failed"));
   return done(MPTC_OK);
 }
 
-int32_t mptc_collect_pairs(mptc_ctx* c, int32_t net, const uint8_t* d_rgb, int32_t width, int32_t height,
-                           const uint8_t* d_motion, const uint8_t* d_eps, int32_t S, uint8_t* d_px, int32_t* d_target,
-                           uint8_t* d_eps_out, uint32_t* d_count, int32_t capacity) {
+int32_t mptc_record_pairs(mptc_ctx* c, int32_t enable) {
+  if (!c) return fail(MPTC_EINVAL, "null context");
+  HIP_TRY(hipSetDevice(c->cfg.device));
+  if (enable && !c->d_amask[0]) {
+    int rc;
+    const size_t nb = (size_t)c->max_total * 16;
+    if ((rc = dalloc(&c->d_amask[0], nb)) || (rc = dalloc(&c->d_amask[1], nb))) return rc;
+  }
+  if ((enable != 0) != c->record) c->graphs.destroy();
+  c->record = enable != 0;
+  c->last_recorded = false;
+  return MPTC_OK;
+}
+
+int32_t mptc_collect_pairs(mptc_ctx* c, int32_t net, uint8_t* d_px, uint32_t* d_target, uint8_t* d_eps_out,
+                           uint32_t* d_count, int32_t capacity) {
   int rc = check_net(c, net);
   if (rc) return rc;
-  if (!d_rgb || !d_motion || !d_eps || !d_px || !d_target || !d_eps_out || !d_count || capacity < 0)
-    return fail(MPTC_EINVAL, "bad argument");
-  if (width <= 0 || height <= 0 || (width & 3) || (height & 3))
-    return fail(MPTC_EINVAL, "width and height must be positive multiples of 4");
-  if (S < 1 || S > kMaxS) return fail(MPTC_EINVAL, "search_area must be in 1..63");
-  const int bw = width / 4, nb = bw * (height / 4);
+  if (!d_px || !d_target || !d_eps_out || !d_count || capacity < 0) return fail(MPTC_EINVAL, "bad argument");
+  if (!c->last_recorded) return fail(MPTC_ESTATE, "the last step did not record acceptance sets");
   hipStream_t s = c->stream;
-  hipLaunchKernelGGL(k_collect_pairs, dim3(grid_for(nb, 256)), dim3(256), 0, s, d_rgb, width, bw, nb, d_motion, d_eps,
-                     S, net, d_px, d_target, d_eps_out, d_count, capacity);
+  HIP_TRY(hipStreamSynchronize(s));  // the step may have run on a caller's stream
+  hipLaunchKernelGGL(k_collect_masks, dim3(grid_for(c->total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, c->total,
+                     net, c->cfg.search_area, c->d_need, c->d_amask[net], c->d_own, d_px, d_target, d_eps_out,
+                     d_count, capacity);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s));
   return MPTC_OK;
@@ -1207,13 +1272,25 @@ int32_t mptc_set_search_mode(mptc_ctx* c, int32_t mode, int32_t top_k) {
     int rc;
     const size_t nb = (size_t)c->max_total;
     if ((rc = work_ensure(c->ew, c->max_total, false))) return rc;
-    if (!c->d_yt && ((rc = dalloc(&c->d_yt, nb * kNetOut)) || (rc = dalloc(&c->d_prop, nb * kMaxTopK)) ||
-                     (rc = dalloc(&c->d_seen, nb * kMaxTopK)) || (rc = dalloc(&c->d_errs, nb * kMaxTopK))))
+    if (!c->d_h2p && ((rc = dalloc(&c->d_h2p, (nb + 31) / 32 * (kH2TileBytes / 16))) ||
+                      (rc = dalloc(&c->d_pinter, nb * kMaxTopK)) || (rc = dalloc(&c->d_prop, nb * kMaxTopK)) ||
+                      (rc = dalloc(&c->d_seen, nb * kMaxTopK)) || (rc = dalloc(&c->d_errs, nb * kMaxTopK))))
       return rc;
   }
   if (mode != c->mode || top_k != c->top_k) c->graphs.destroy();
   c->mode = mode;
   c->top_k = top_k;
+  return MPTC_OK;
+}
+
+int32_t mptc_get_proposals(mptc_ctx* c, int32_t net, int16_t* out, int64_t n_blocks) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  if (!out || n_blocks < 0 || n_blocks > c->max_total) return fail(MPTC_EINVAL, "bad argument");
+  const int16_t* src = net == MPTC_NET_INTER ? c->d_pinter : c->d_prop;
+  if (!src) return fail(MPTC_ESTATE, "no learned step has run");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(out, src, sizeof(int16_t) * (size_t)n_blocks * c->top_k, hipMemcpyDeviceToHost));
   return MPTC_OK;
 }
 

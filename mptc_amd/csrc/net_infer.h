@@ -1,0 +1,372 @@
+// Inference kernels of the proposal network (mlp_kernels.h): two fused
+// kernels whose weights stay resident in LDS for the whole launch.
+//
+//   k_net_l12   x -> h1 -> h2: layers 1 and 2 in registers, h2 stored to HBM
+//               in the operand order layer 3 reads
+//   k_net_l3    h2 -> y: layer 3, then either the 262 outputs (row-major
+//               fp32) or, fused in the epilogue, each block's top-k window
+//               offsets among those its search admits
+//
+// MFMA v_mfma_f32_32x32x16_bf16 computes D[n][m] = sum_k A[n][k] B[k][m] with
+// A = W^T (32 outputs n x 16 inputs k), B = activations^T (16 k x 32 blocks m).
+// Lane l (h = l / 32) holds A row l % 32, k = 8h .. 8h+7; B column l % 32,
+// k = 8h .. 8h+7; D column l % 32, rows 8(i/4) + 4h + i%4, i = 0..15.
+// A layer's D tiles become the next layer's B operands without leaving the
+// registers: for k-step s, lane half h takes the 8 D values i = 8(s%2) .. +7
+// of tile s/2, which hold hidden unit f(s, h, j) = 16s + 8(j/4) + 4h + j%4
+// (j = 0..7).  The next layer's weights are stored with their input index
+// permuted the same way (position p = 16s + 8h + j holds input f(s, h, j)),
+// so its A fragments are contiguous 16-byte LDS reads.  Rows of the LDS
+// weight images are split in 16-byte chunks stored at chunk ^ (row % 8), so
+// the 8 lanes of an LDS read phase hit 8 different bank groups.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mlp_kernels.h"
+
+namespace mptc {
+
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
+
+constexpr int kL3Rows = 288;  // 262 outputs padded to 9 tiles of 32
+// packed inference image (bf16 units): W1^T [256][48], W2^T [256][256] and
+// W3^T [288][256] with permuted inputs; then fp32 biases b1, b2, b3 (288)
+constexpr int kPkW1 = 0;
+constexpr int kPkW2 = kPkW1 + kNetHid * kNetIn;
+constexpr int kPkW3 = kPkW2 + kNetHid * kNetHid;
+constexpr int kPkBias = kPkW3 + kL3Rows * kNetHid;         // bf16 offset of the fp32 biases
+constexpr int kPkSize = kPkBias + 2 * (2 * kNetHid + kL3Rows);  // bf16 units
+constexpr int kH2TileBytes = 16 * 64 * 16;                // one 32-block tile of h2 operands
+
+// hidden unit held at permuted input position p
+__host__ __device__ __forceinline__ int perm_unit(int p) {
+  const int s = p >> 4, h = (p >> 3) & 1, j = p & 7;
+  return 16 * s + 8 * (j >> 2) + 4 * h + (j & 3);
+}
+
+// fp32 master parameters -> packed inference image.
+__global__ void __launch_bounds__(256) k_pack_infer(const float* __restrict__ w, uint16_t* __restrict__ pk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kNetHid * kNetIn) {  // W1^T[n][k] = W1[k][n]
+    const int n = i / kNetIn, k = i % kNetIn;
+    pk[kPkW1 + i] = f2bf(w[kOffW1 + k * kNetHid + n]);
+  }
+  if (i < kNetHid * kNetHid) {  // W2^T[n][p] = W2[f(p)][n]
+    const int n = i / kNetHid, p = i % kNetHid;
+    pk[kPkW2 + i] = f2bf(w[kOffW2 + perm_unit(p) * kNetHid + n]);
+  }
+  if (i < kL3Rows * kNetHid) {  // W3^T[n][p] = W3[f(p)][n], rows >= 262 zero
+    const int n = i / kNetHid, p = i % kNetHid;
+    pk[kPkW3 + i] = n < kNetOut ? f2bf(w[kOffW3 + perm_unit(p) * kNetOut + n]) : (uint16_t)0;
+  }
+  float* bias = reinterpret_cast<float*>(pk + kPkBias);
+  if (i < kNetHid) {
+    bias[i] = w[kOffB1 + i];
+    bias[kNetHid + i] = w[kOffB2 + i];
+  }
+  if (i < kL3Rows) bias[2 * kNetHid + i] = i < kNetOut ? w[kOffB3 + i] : 0.0f;
+}
+
+// Copies `rows` rows of `chunks` 16-byte chunks into LDS, chunk c of row r at
+// r * chunks + (c ^ (r & 7)) (swizzled) or r * chunks + c.
+__device__ __forceinline__ void stage_rows(uint4* __restrict__ dst, const uint4* __restrict__ src, int rows,
+                                           int chunks, bool swz) {
+  for (int e = threadIdx.x; e < rows * chunks; e += blockDim.x) {
+    const int r = e / chunks, c = e % chunks;
+    dst[r * chunks + (swz ? (c ^ (r & 7)) : c)] = src[e];
+  }
+}
+
+__device__ __forceinline__ bf16x8_t as_bf16x8(uint4 v) {
+  bf16x8_t r;
+  __builtin_memcpy(&r, &v, 16);
+  return r;
+}
+
+// D values i = 8*half .. +7 of a tile, plus bias, ReLU, as a bf16 B operand.
+__device__ __forceinline__ uint4 relu_pack(const f32x16_t& d, int half, const float* __restrict__ sb, int nb,
+                                           int h) {
+  uint32_t w[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i0 = 8 * half + 2 * q, i1 = i0 + 1;
+    const int u0 = nb * 32 + 8 * (i0 >> 2) + 4 * h + (i0 & 3), u1 = nb * 32 + 8 * (i1 >> 2) + 4 * h + (i1 & 3);
+    const float v0 = fmaxf(d[i0] + sb[u0], 0.0f), v1 = fmaxf(d[i1] + sb[u1], 0.0f);
+    w[q] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+  }
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+constexpr int kInferWaves = 8;  // waves per workgroup: two per SIMD share the LDS weights
+
+// Layers 1 and 2.  X: rows x 48 bf16; H2p: per 32-block tile, per k-step s,
+// per lane, the 8 bf16 layer-3 operand values (kH2TileBytes per tile).
+// One workgroup per CU; each wave loops over tiles.
+__global__ void __launch_bounds__(64 * kInferWaves) k_net_l12(const uint16_t* __restrict__ X, int rows,
+                                                 const uint16_t* __restrict__ pk, uint4* __restrict__ H2p) {
+  __shared__ uint4 sW1[kNetHid * kNetIn / 8];   // 24 KiB, rows of 6 chunks
+  __shared__ uint4 sW2[kNetHid * kNetHid / 8];  // 128 KiB, rows of 32 chunks, swizzled
+  __shared__ float sb[2 * kNetHid];
+  stage_rows(sW1, reinterpret_cast<const uint4*>(pk + kPkW1), kNetHid, kNetIn / 8, false);
+  stage_rows(sW2, reinterpret_cast<const uint4*>(pk + kPkW2), kNetHid, kNetHid / 8, true);
+  const float* bias = reinterpret_cast<const float*>(pk + kPkBias);
+  for (int i = threadIdx.x; i < 2 * kNetHid; i += blockDim.x) sb[i] = bias[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, m = lane & 31, h = lane >> 5;
+  const int n_tiles = (rows + 31) / 32;
+  const int wave_g = blockIdx.x * kInferWaves + (threadIdx.x >> 6);
+  for (int tile = wave_g; tile < n_tiles; tile += gridDim.x * kInferWaves) {
+    const int row = tile * 32 + m;
+    // layer 1: B operands straight from X (k = 16s + 8h .. +7)
+    bf16x8_t xb[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const uint4 v = row < rows ? *reinterpret_cast<const uint4*>(X + (size_t)row * kNetIn + 16 * s + 8 * h)
+                                 : make_uint4(0, 0, 0, 0);
+      xb[s] = as_bf16x8(v);
+    }
+    bf16x8_t hb[16];
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+      f32x16_t acc = {};
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const bf16x8_t a = as_bf16x8(sW1[(nb * 32 + m) * 6 + 2 * s + h]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xb[s], acc, 0, 0, 0);
+      }
+      hb[2 * nb] = as_bf16x8(relu_pack(acc, 0, sb, nb, h));
+      hb[2 * nb + 1] = as_bf16x8(relu_pack(acc, 1, sb, nb, h));
+      __builtin_amdgcn_sched_barrier(0);  // one accumulator live at a time
+    }
+    // layer 2 (one output tile at a time keeps the live registers to h1 + one accumulator)
+    uint4* out = H2p + (size_t)tile * (kH2TileBytes / 16);
+#pragma unroll 1
+    for (int nb = 0; nb < 8; ++nb) {
+      f32x16_t acc = {};
+      const int r = nb * 32 + m;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const bf16x8_t a = as_bf16x8(sW2[r * 32 + ((2 * s + h) ^ (r & 7))]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, hb[s], acc, 0, 0, 0);
+      }
+      out[(2 * nb) * 64 + lane] = relu_pack(acc, 0, sb + kNetHid, nb, h);
+      out[(2 * nb + 1) * 64 + lane] = relu_pack(acc, 1, sb + kNetHid, nb, h);
+    }
+  }
+}
+
+// Which window offsets a block's search admits: the offsets of the 16x16
+// grid within the search area and the frame, and for the intra network only
+// causal ones (rows dy > 0, or dy = 0 with dx < 0).  Bit r of `rows`, bit c
+// of `cols` (row 0: `cols0`) admit logit L = 16 r + c.
+struct WinMask {
+  uint32_t rows, cols, cols0;
+  __device__ __forceinline__ bool operator()(int L) const {
+    const int r = L >> 4, c = L & 15;
+    return ((rows >> r) & ((r == 0 ? cols0 : cols) >> c) & 1u) != 0;
+  }
+};
+
+__device__ __forceinline__ WinMask win_mask(int net, int S, int bx, int by, int bw, int bh) {
+  const int lo = -min(S, 8), hi = min(S, 8) - 1;
+  WinMask m{0u, 0u, 0u};
+  for (int c = 0; c < kNetWin; ++c) {
+    const int dx = c - 8;
+    if (dx >= lo && dx <= hi && bx + dx >= 0 && bx + dx < bw) m.cols |= 1u << c;
+  }
+  for (int r = 0; r < kNetWin; ++r) {
+    bool ok;
+    if (net == 0) {
+      const int dy = r - 8;
+      ok = dy >= lo && dy <= hi && by + dy >= 0 && by + dy < bh;
+    } else {
+      ok = r <= min(2 * S - 1, 15) && by - r >= 0;
+    }
+    if (ok) m.rows |= 1u << r;
+  }
+  m.cols0 = net == 0 ? m.cols : (m.cols & 0xFFu);  // intra row 0: dx < 0
+  return m;
+}
+
+// Rank key of an admitted offset's score: the fp32 score mapped to an
+// order-preserving unsigned integer, its low 8 bits replaced by 255 - L.  A
+// larger key ranks first; scores equal to 24 bits rank by the lower index; a
+// NaN score ranks below every number; 0 means no offset.
+__device__ __forceinline__ uint32_t rank_key(float v, int L) {
+  uint32_t u = __float_as_uint(v);
+  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  if (v != v) u = 0x100u;
+  return (u & 0xFFFFFF00u) | (uint32_t)(255 - L);
+}
+
+__device__ __forceinline__ int key_offset(uint32_t key) { return key ? 255 - (int)(key & 0xFFu) : -1; }
+
+// Inserts a key into a list sorted in descending order (branch-free: every
+// lane of the wave does the same work, whatever its data).
+template <int K>
+__device__ __forceinline__ void key_insert(uint32_t (&t)[K], uint32_t key) {
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const uint32_t hi = max(t[i], key);
+    key = min(t[i], key);
+    t[i] = hi;
+  }
+}
+
+// Layer 3.  With Y non-null: writes the 262 outputs of each row (row-major).
+// Otherwise: for each block g of the batch (rows = total) that `need`
+// admits (need null: every block of an inter frame), the top kk <= K
+// admitted offsets by score go to prop[g * kk + i] (-1 past the last).
+template <int K>
+__global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __restrict__ H2p, int rows,
+                                                             const uint16_t* __restrict__ pk, float* __restrict__ Y,
+                                                             const FrameDev* __restrict__ fr,
+                                                             const uint8_t* __restrict__ sid, int net, int S, int kk,
+                                                             const uint8_t* __restrict__ need,
+                                                             int16_t* __restrict__ prop) {
+  __shared__ uint4 sW3[kL3Rows * kNetHid / 8];  // 144 KiB, rows of 32 chunks, swizzled
+  __shared__ float sb[kL3Rows];
+  stage_rows(sW3, reinterpret_cast<const uint4*>(pk + kPkW3), kL3Rows, kNetHid / 8, true);
+  const float* bias = reinterpret_cast<const float*>(pk + kPkBias) + 2 * kNetHid;
+  for (int i = threadIdx.x; i < kL3Rows; i += blockDim.x) sb[i] = bias[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, m = lane & 31, h = lane >> 5;
+  const int n_tiles = (rows + 31) / 32;
+  const int wave_g = blockIdx.x * kInferWaves + (threadIdx.x >> 6);
+  for (int tile = wave_g; tile < n_tiles; tile += gridDim.x * kInferWaves) {
+    const int row = tile * 32 + m;
+    f32x16_t acc[9];
+#pragma unroll
+    for (int nb = 0; nb < 9; ++nb) acc[nb] = f32x16_t{};
+    const uint4* in = H2p + (size_t)tile * (kH2TileBytes / 16);
+#pragma unroll 2
+    for (int s = 0; s < 16; ++s) {
+      const bf16x8_t b = as_bf16x8(in[s * 64 + lane]);
+#pragma unroll
+      for (int nb = 0; nb < 9; ++nb) {
+        const int r = nb * 32 + m;
+        const bf16x8_t a = as_bf16x8(sW3[r * 32 + ((2 * s + h) ^ (r & 7))]);
+        acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nb], 0, 0, 0);
+      }
+    }
+    if (Y) {
+      if (row < rows) {
+#pragma unroll
+        for (int nb = 0; nb < 9; ++nb)
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const int u = nb * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+            if (u < kNetOut) Y[(size_t)row * kNetOut + u] = acc[nb][i] + sb[u];
+          }
+      }
+      continue;
+    }
+    // fused top-k: each lane ranks its 128 logits; lane h = 0 merges the
+    // partner lane's (l + 32) list and writes
+    bool act = row < rows;
+    WinMask ok{0u, 0u, 0u};
+    if (act) {
+      const FrameDev& d = fr[sid[row]];
+      const int k = row - d.base;
+      act = need ? need[row] != 0 : !d.is_intra;
+      if (act) ok = win_mask(net, S, k % d.bw, k / d.bw, d.bw, d.bh);
+    }
+    uint32_t t[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) t[i] = 0u;
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int L = nb * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+        key_insert<K>(t, ok(L) ? rank_key(acc[nb][i] + sb[L], L) : 0u);
+      }
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const uint32_t pk2 = __shfl_xor(t[i], 32);
+      if (h == 0) key_insert<K>(t, pk2);
+    }
+    if (h == 0 && act) {
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        if (i < kk) prop[(size_t)row * kk + i] = (int16_t)key_offset(t[i]);
+    }
+  }
+}
+
+// Top-k admitted offsets from fixed scores (one shared prior, ys_g = 0) or
+// transposed logits (ys_l = rows, ys_g = 1), ranked as k_net_l3's fused top-k.
+__global__ void __launch_bounds__(256) k_top_scores(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
+                                                    int total, int net, int S, int kk, const float* __restrict__ Ys,
+                                                    long ys_l, long ys_g, const uint8_t* __restrict__ need,
+                                                    int16_t* __restrict__ prop) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const FrameDev& d = fr[sid[g]];
+  if (need ? !need[g] : d.is_intra) return;
+  const int k = g - d.base;
+  const WinMask ok = win_mask(net, S, k % d.bw, k / d.bw, d.bw, d.bh);
+  uint32_t t[kMaxTopK];
+#pragma unroll
+  for (int i = 0; i < kMaxTopK; ++i) t[i] = 0u;
+  const float* y = Ys + (long)g * ys_g;
+  for (int L = 0; L < kNetLogits; ++L)
+    if (ok(L)) key_insert<kMaxTopK>(t, rank_key(y[(long)L * ys_l], L));
+#pragma unroll
+  for (int i = 0; i < kMaxTopK; ++i)
+    if (i < kk) prop[(size_t)g * kk + i] = (int16_t)key_offset(t[i]);
+}
+
+// Learned inter search: each proposed offset is evaluated exactly (the same
+// re-fit and error as InterBlockSearch, codec/dxt_image.cpp:715-774); the
+// valid one with the smallest error delta (first in rank order on ties) is
+// reused if the delta passes the threshold (:890).
+__global__ void __launch_bounds__(256) k_eval_inter(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
+                                                    int total, int S, int thr, int kk,
+                                                    const int16_t* __restrict__ prop,
+                                                    const uint64_t* __restrict__ own, const uint32_t* __restrict__ rw,
+                                                    const float4* __restrict__ rtab, uint64_t* __restrict__ dec_out,
+                                                    DevStats* __restrict__ st) {
+  __shared__ uint8_t s_snap[512];
+  __shared__ float2 s_ab[4];
+  __shared__ unsigned long long s_acc;
+  load_lds_tables(s_snap, s_ab);
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long n_cand = 0;
+  if (g < total && !fr[sid[g]].is_intra) {
+    const FrameDev& d = fr[sid[g]];
+    const int k = g - d.base, bx = k % d.bw, by = k / d.bw, bw = d.bw;
+    BlockPx px;
+    load_block(d.rgb, d.width, bx, by, px);
+    BlockEval be;
+    make_block_eval(px, own[g], be);
+    int best = 0x7fffffff, best_L = -1;
+    uint32_t best_c = 0;
+#pragma unroll
+    for (int i = 0; i < kMaxTopK; ++i) {
+      const int L = i < kk ? (int)prop[(size_t)g * kk + i] : -1;
+      if (L >= 0) {
+        const int q = d.base + (by + L / kNetWin - 8) * bw + (bx + L % kNetWin - 8);
+        const float4 t4 = rtab[q];
+        const uint32_t c = rw[q];
+        const CandResult res = eval_candidate<false>(be, c, CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap, s_ab);
+        ++n_cand;
+        if (res.valid && res.err < best) {
+          best = res.err;
+          best_L = L;
+          best_c = c;
+        }
+      }
+    }
+    uint64_t dc = 0;
+    if (best_L >= 0 && best <= thr)
+      dc = pack_dec(best_c, (uint32_t)(best_L % kNetWin - 8 + S) | 0x80u, (uint32_t)(best_L / kNetWin - 8 + S) | 0x80u,
+                    true);
+    dec_out[g] = dc;
+  }
+  wg_count(&s_acc, n_cand, st->inter_cands);
+}
+
+}  // namespace mptc

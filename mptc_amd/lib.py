@@ -113,10 +113,11 @@ def load() -> C.CDLL:
     L.mptc_net_gradients.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.c_float, _vp,
                                      C.POINTER(C.c_float)]
     L.mptc_net_train.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.POINTER(TrainCfg), _vp]
-    L.mptc_collect_pairs.argtypes = [_vp, C.c_int32, _vp, C.c_int32, C.c_int32, _vp, _vp, C.c_int32,
-                                     _vp, _vp, _vp, _vp, C.c_int32]
+    L.mptc_record_pairs.argtypes = [_vp, C.c_int32]
+    L.mptc_collect_pairs.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, C.c_int32]
     L.mptc_set_search_mode.argtypes = [_vp, C.c_int32, C.c_int32]
     L.mptc_set_prior.argtypes = [_vp, C.c_int32, _vp]
+    L.mptc_get_proposals.argtypes = [_vp, C.c_int32, _vp, C.c_int64]
     for name in header_functions():
         if name not in ("mptc_destroy", "mptc_last_error"):
             getattr(L, name).restype = C.c_int32
@@ -296,14 +297,22 @@ class Encoder:
                                      C.byref(cfg), losses.ctypes.data))
         return losses
 
-    def collect_pairs(self, net: int, d_rgb, w: int, h: int, d_motion, d_eps, search_area: int,
-                      d_px, d_target, d_eps_out, d_count, capacity: int):
-        _check(self.L.mptc_collect_pairs(self.h, net, _ptr(d_rgb), w, h, _ptr(d_motion), _ptr(d_eps),
-                                         search_area, _ptr(d_px), _ptr(d_target), _ptr(d_eps_out),
+    def record_pairs(self, on: bool = True):
+        _check(self.L.mptc_record_pairs(self.h, int(on)))
+
+    def collect_pairs(self, net: int, d_px, d_target, d_eps_out, d_count, capacity: int):
+        """Appends the last (recording, exhaustive) step's pairs for `net`:
+        pixels (48 B), target set (8 x uint32), endpoints (6 B) per pair."""
+        _check(self.L.mptc_collect_pairs(self.h, net, _ptr(d_px), _ptr(d_target), _ptr(d_eps_out),
                                          _ptr(d_count), capacity))
 
     def set_search_mode(self, mode: int, top_k: int = 8):
         _check(self.L.mptc_set_search_mode(self.h, mode, top_k))
+
+    def proposals(self, net: int, n_blocks: int, top_k: int) -> np.ndarray:
+        out = np.zeros(n_blocks * top_k, np.int16)
+        _check(self.L.mptc_get_proposals(self.h, net, out.ctypes.data, n_blocks))
+        return out.reshape(n_blocks, top_k)
 
     def set_prior(self, net: int, scores: np.ndarray | None):
         if scores is None:

@@ -11,8 +11,9 @@ follow.  Its definition is the one include/mptc.h states:
   h1 = bf16(relu(x  @ bf16(W1) + b1))                 256
   h2 = bf16(relu(h1 @ bf16(W2) + b2))                 256
   y  = h2 @ bf16(W3) + b3                             262: 256 logits, 6 seeds
-  loss = mean over rows of [ CE(softmax(y[:256]), target)
-                             + seed_w * mean_c (y[256 + c] - eps[c] / 255)^2 ]
+  loss = mean over rows of [ -log sum_{L in T} softmax(y[:256])_L + seed_w * mean_c (y[256 + c] - eps[c] / 255)^2 ]
+  T  = the row's target set (one index = the usual cross-entropy; an empty set
+       contributes only the seed term)
 
 with W[k][n] (input k, output n) and FP32 biases.  The backward pass rounds
 dY, dH2 and dH1 to bf16 before they enter a product, as the GPU kernels do:
@@ -84,7 +85,21 @@ def forward(params: np.ndarray, px: np.ndarray, exact: bool = False) -> dict:
     return dict(X=X, H1=H1, H2=H2, Y=Y, W1=W1, W2=W2, W3=W3)
 
 
-def loss_grad(params: np.ndarray, px: np.ndarray, target: np.ndarray, eps: np.ndarray,
+def target_sets(target, n: int) -> np.ndarray:
+    """n x 256 booleans: from logit indices (one per row), from n x 256
+    booleans, or from n x 8 uint32 words (bit L of word L // 32)."""
+    t = np.asarray(target)
+    if t.ndim == 1:
+        m = np.zeros((n, N_LOGITS), bool)
+        m[np.arange(n), t.astype(np.int64)] = True
+        return m
+    if t.shape[1] == 8:
+        bits = (t.astype(np.uint64)[:, np.arange(N_LOGITS) // 32] >> (np.arange(N_LOGITS) % 32).astype(np.uint64)) & 1
+        return bits.astype(bool)
+    return t.astype(bool)
+
+
+def loss_grad(params: np.ndarray, px: np.ndarray, target, eps: np.ndarray,
               seed_w: float, exact: bool = False):
     """Mean loss over the rows and its gradient (flat, parameter layout)."""
     f = forward(params, px, exact)
@@ -94,14 +109,17 @@ def loss_grad(params: np.ndarray, px: np.ndarray, target: np.ndarray, eps: np.nd
     mx = logits.max(axis=1, keepdims=True)
     lse = np.log(np.exp(logits - mx).sum(axis=1, keepdims=True))
     prob = np.exp(logits - mx - lse)
-    rows = np.arange(n)
-    t = np.asarray(target, np.int64)
+    T = target_sets(target, n)
     seeds_t = np.asarray(eps, np.float64).reshape(n, N_SEEDS) * (1.0 / 255.0)
     d = Y[:, N_LOGITS:] - seeds_t
-    loss = float(np.mean(lse[:, 0] - (logits[rows, t] - mx[:, 0]) + seed_w * (d * d).mean(axis=1)))
+    ex = np.exp(logits - mx)
+    ss = (ex * T).sum(axis=1, keepdims=True)
+    has = ss[:, 0] > 0
+    set_term = np.where(has, lse[:, 0] - np.log(np.where(has, ss[:, 0], 1.0)), 0.0)
+    loss = float(np.mean(set_term + seed_w * (d * d).mean(axis=1)))
+    r = np.divide(ex * T, ss, out=np.zeros_like(ex), where=ss > 0)
     dY = np.zeros_like(Y)
-    dY[:, :N_LOGITS] = prob
-    dY[rows, t] -= 1.0
+    dY[:, :N_LOGITS] = np.where(has[:, None], prob - r, 0.0)
     dY[:, N_LOGITS:] = 2.0 * seed_w * d / N_SEEDS
     dY /= n
     dYr = _r(dY, exact)

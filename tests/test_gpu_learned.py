@@ -145,24 +145,96 @@ def test_learned_mode_rejects_missing_networks(env):
 
 
 def _collect(enc, lib, hip, frames, S, thr, cap):
-    """Plain-search training pairs of both networks from a sequence."""
+    """Exhaustive-search training pairs of both networks from a sequence."""
     seq = Seq(hip, frames)
     bufs = {}
     for net in (0, 1):
-        bufs[net] = dict(px=hip.Buffer(48 * cap), tgt=hip.Buffer(4 * cap), eps=hip.Buffer(6 * cap),
+        bufs[net] = dict(px=hip.Buffer(48 * cap), tgt=hip.Buffer(32 * cap), eps=hip.Buffer(6 * cap),
                          count=hip.Buffer.from_array(np.zeros(1, np.uint32)))
 
     def grab(f):
         for net in (0, 1):
             b = bufs[net]
-            enc.collect_pairs(net, seq.rgb_at(f), seq.w, seq.h, seq.motion[f], seq.eps[f], S, b["px"], b["tgt"],
-                              b["eps"], b["count"], cap)
+            enc.collect_pairs(net, b["px"], b["tgt"], b["eps"], b["count"], cap)
 
     enc.set_search_mode(0, 8)
+    enc.record_pairs(True)
     encode(enc, lib, seq, 8, collect=grab)
+    enc.record_pairs(False)
     for net in (0, 1):
         bufs[net]["n"] = min(cap, int(bufs[net]["count"].download(np.uint32, 1)[0]))
     return bufs
+
+
+@pytest.mark.parametrize("S,thr", [(8, 0), (8, 150), (5, 40)])
+def test_recorded_sets_match_the_accept_test(env, orc, S, thr):
+    """Every offset of a recorded set passes the accept test by the oracle and
+    every passing offset is recorded; blocks are told apart by their noise
+    pixels.  With thr >= 0 a block has a non-empty inter (intra) set exactly
+    when the search reused an inter (intra) word."""
+    hip, lib = env
+    rng = np.random.default_rng(S * 1000 + thr)
+    h, w = 64, 96
+    base = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    f1 = np.clip(base.astype(int) + rng.integers(-6, 7, (h, w, 3)), 0, 255).astype(np.uint8)
+    # flat patches so that some words repeat and pass at thr = 0
+    base[8:24, 8:40] = 120
+    f1[8:24, 12:44] = 121
+    frames = np.stack([base, f1])
+    seq = Seq(hip, frames)
+    enc = lib.Encoder(w, h, S, thr)
+    enc.record_pairs(True)
+    bw, bh = w // 4, h // 4
+    nb = bw * bh
+    cap = nb
+    kinds = hip.Buffer(nb)
+    for f in range(2):
+        enc.encode_frame_device(seq.rgb_at(f), w, h, f == 0, None if f == 0 else seq.phys[0], seq.phys[f],
+                                seq.motion[f], seq.unique, seq.n_unique, seq.eps[f], kinds)
+    st = enc.stats()
+    kind = kinds.download(np.uint8, nb)
+    own = orc.fit_frame(f1)
+    ref = seq.phys[0].download(np.uint64, nb)
+    out = seq.phys[1].download(np.uint64, nb)
+    blocks = {f1[4 * by:4 * by + 4, 4 * bx:4 * bx + 4].tobytes(): (bx, by) for by in range(bh) for bx in range(bw)}
+    assert len(blocks) > nb // 2
+    for net in (0, 1):
+        px, tgt, eps = hip.Buffer(48 * cap), hip.Buffer(32 * cap), hip.Buffer(6 * cap)
+        cnt = hip.Buffer.from_array(np.zeros(1, np.uint32))
+        enc.collect_pairs(net, px, tgt, eps, cnt, cap)
+        n = int(cnt.download(np.uint32, 1)[0])
+        assert n == (st["n_inter"] if net == 0 else st["n_intra"]), (net, n, st)
+        P = px.download(np.uint8, 48 * n).reshape(n, 48)
+        T = tgt.download(np.uint32, 8 * n).reshape(n, 8)
+        checked = 0
+        for p in range(n):
+            key = P[p].reshape(4, 4, 3).tobytes()
+            if key not in blocks:
+                continue
+            bx, by = blocks[key]
+            k = by * bw + bx
+            if f1[4 * by:4 * by + 4, 4 * bx:4 * bx + 4].reshape(-1).tolist().count(121) > 40:
+                continue  # flat patch: pixels do not identify the block
+            exp = np.zeros(256, bool)
+            for L in range(256):
+                dx = L % 16 - 8
+                dy = L // 16 - 8 if net == 0 else L // 16
+                j, i = (by + dy, bx + dx) if net == 0 else (by - dy, bx + dx)
+                if not (-S <= dx < S) or not (0 <= i < bw and 0 <= j < bh):
+                    continue
+                if net == 0 and not (-S <= dy < S):
+                    continue
+                if net == 1 and (dy > 2 * S - 1 or (dy == 0 and dx >= 0)):
+                    continue
+                word = int((ref if net == 0 else out)[j * bw + i]) >> 32
+                ok, e, _, _ = orc.eval_candidate(int(own[k]), f1[4 * by:4 * by + 4, 4 * bx:4 * bx + 4], word)
+                exp[L] = ok and e <= thr
+            got = np.array([(int(T[p, L // 32]) >> (L % 32)) & 1 for L in range(256)], bool)
+            assert np.array_equal(got, exp), (net, bx, by, np.flatnonzero(got), np.flatnonzero(exp))
+            checked += 1
+        log(f"net {net}: {n} pairs, {checked} checked")
+        assert n == 0 or checked > 0
+    enc.close()
 
 
 def test_trained_network_beats_untrained_and_fixed_proposals(env, orc):
@@ -206,4 +278,66 @@ def test_trained_network_beats_untrained_and_fixed_proposals(env, orc):
     log(f"reused blocks: plain {plain}, untrained {untrained}, nearest-first {fixed}, trained {trained}")
     assert trained > untrained
     assert trained >= 0.5 * plain
+    enc.close()
+
+
+def rank_keys(y, L):
+    """The proposal ranking: fp32 scores as order-preserving integers with the
+    low 8 bits replaced by 255 - L (larger first)."""
+    u = np.asarray(y, np.float32).view(np.uint32).astype(np.int64)
+    u = np.where(u & 0x80000000, (~u) & 0xFFFFFFFF, u | 0x80000000)
+    u = np.where(np.isnan(y), 0x100, u)
+    return (u & 0xFFFFFF00) | (255 - np.asarray(L, np.int64))
+
+
+def _admitted(net, S, bx, by, bw, bh):
+    """Offsets (logit indices) the search of block (bx, by) admits."""
+    L = np.arange(256)
+    dx = L % 16 - 8
+    lo, hi = -min(S, 8), min(S, 8) - 1
+    ok = (dx >= lo) & (dx <= hi) & (bx + dx >= 0) & (bx + dx < bw)
+    if net == 0:
+        dy = L // 16 - 8
+        return ok & (dy >= lo) & (dy <= hi) & (by + dy >= 0) & (by + dy < bh)
+    dy = L // 16
+    return ok & (dy <= min(2 * S - 1, 15)) & (by - dy >= 0) & ((dy > 0) | (dx < 0))
+
+
+@pytest.mark.parametrize("k,S", [(8, 8), (3, 5), (16, 8)])
+def test_fused_topk_matches_network_logits(env, k, S):
+    """The step's proposals are the top-k admitted logits of the network
+    (ties to the lower index), as computed from the standalone forward pass."""
+    hip, lib = env
+    spec = dict(SEQUENCES["c1_s8_t8"])
+    frames = lib.synth_frames(synth_cfg(spec), 0, 2)
+    seq = Seq(hip, frames)
+    enc = lib.Encoder(seq.w, seq.h, S, 0)
+    for net in (0, 1):
+        enc.net_init(net, 31 + net)
+    enc.set_search_mode(2, k)
+    kinds = hip.Buffer(seq.nb)
+    for f in range(2):
+        enc.encode_frame_device(seq.rgb_at(f), seq.w, seq.h, f == 0, None if f == 0 else seq.phys[0], seq.phys[f],
+                                seq.motion[f], seq.unique, seq.n_unique, seq.eps[f], kinds)
+    kind = kinds.download(np.uint8, seq.nb)
+    bw, bh = seq.w // 4, seq.h // 4
+    px = frames[1].reshape(bh, 4, bw, 4, 3).transpose(0, 2, 1, 3, 4).reshape(seq.nb, 48).copy()
+    d_px, d_y = hip.Buffer.from_array(px), hip.Buffer(seq.nb * 262 * 4)
+    for net in (0, 1):
+        props = enc.proposals(net, seq.nb, k)
+        enc.net_forward(net, d_px, seq.nb, d_y)
+        y = d_y.download(np.float32, seq.nb * 262).reshape(seq.nb, 262)[:, :256]
+        checked = 0
+        for g in range(seq.nb):
+            if net == 1 and kind[g] == 2:
+                continue  # inter-reused blocks do not search the intra window
+            bx, by = g % bw, g // bw
+            ok = _admitted(net, S, bx, by, bw, bh)
+            cand = np.flatnonzero(ok)
+            order = cand[np.argsort(-rank_keys(y[g, cand], cand), kind="stable")][:k]
+            exp = np.full(k, -1)
+            exp[:len(order)] = order
+            assert np.array_equal(props[g], exp), (net, g, props[g], exp)
+            checked += 1
+        assert checked > 0
     enc.close()

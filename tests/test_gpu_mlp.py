@@ -78,15 +78,33 @@ def test_forward_layout_detects_transposes(env):
     assert np.allclose(y, exp, atol=1e-6), np.abs(y - exp).max()
 
 
-@pytest.mark.parametrize("n,seed_w", [(64, 1.0), (513, 0.25), (3000, 4.0)])
-def test_gradients_match_oracle(env, n, seed_w):
+def set_words(rng, n, density):
+    """Random target sets as n x 8 uint32 words; density 0 = one offset per row."""
+    if density == 0:
+        sets = np.zeros((n, 256), bool)
+        sets[np.arange(n), rng.integers(0, 256, n)] = True
+    else:
+        sets = rng.random((n, 256)) < density
+        sets[np.arange(n), rng.integers(0, 256, n)] = True
+    return np.packbits(sets.reshape(n, 8, 32)[:, :, ::-1], axis=2, bitorder="big").view(">u4").reshape(n, 8).astype(np.uint32)
+
+
+def test_set_words_layout():
+    rng = np.random.default_rng(0)
+    w = set_words(rng, 3, 0.0)
+    q = mlp.target_sets(w, 3)
+    assert np.array_equal((q > 0).sum(axis=1), [1, 1, 1])
+
+
+@pytest.mark.parametrize("n,seed_w,density", [(64, 1.0, 0.0), (513, 0.25, 0.05), (3000, 4.0, 0.3)])
+def test_gradients_match_oracle(env, n, seed_w, density):
     hip, lib, enc = env
     log(f"gradients n={n}")
     rng = np.random.default_rng(100 + n)
     p = mlp.he_params(n)
     enc.net_set_params(0, p)
     px = rng.integers(0, 256, (n, 48), dtype=np.uint8)
-    tgt = rng.integers(0, 256, n).astype(np.int32)
+    tgt = set_words(rng, n, density)
     eps = rng.integers(0, 256, (n, 6), dtype=np.uint8)
     g, loss = enc.net_gradients(0, hip.Buffer.from_array(px), hip.Buffer.from_array(tgt),
                                 hip.Buffer.from_array(eps), n, seed_w)
@@ -103,7 +121,7 @@ def test_adam_step_matches_oracle(env):
     rng = np.random.default_rng(9)
     p = mlp.he_params(9)
     px = rng.integers(0, 256, (n, 48), dtype=np.uint8)
-    tgt = rng.integers(0, 256, n).astype(np.int32)
+    tgt = set_words(rng, n, 0.02)
     eps = rng.integers(0, 256, (n, 6), dtype=np.uint8)
     d = [hip.Buffer.from_array(a) for a in (px, tgt, eps)]
     enc.net_set_params(0, p)
@@ -126,7 +144,10 @@ def test_training_fits_a_learnable_rule(env):
     n = 20000
     rng = np.random.default_rng(4)
     px = rng.integers(0, 256, (n, 48), dtype=np.uint8)
-    tgt = ((px[:, 0] // 16).astype(np.int32) * 16 + 7).astype(np.int32)
+    idx = (px[:, 0] // 16).astype(np.int64) * 16 + 7
+    sets = np.zeros((n, 256), bool)
+    sets[np.arange(n), idx] = True
+    tgt = np.packbits(sets.reshape(n, 8, 32)[:, :, ::-1], axis=2, bitorder="big").view(">u4").reshape(n, 8).astype(np.uint32)
     eps = px[:, :6].copy()
     d = [hip.Buffer.from_array(a) for a in (px, tgt, eps)]
     enc.net_init(1, 5)

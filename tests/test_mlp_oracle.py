@@ -26,18 +26,23 @@ def test_bf16_rounding_matches_torch():
     assert np.isnan(nan[0])
 
 
-def _fd_setup(seed=3, n=6):
+def _fd_setup(seed=3, n=6, sets=False):
     rng = np.random.default_rng(seed)
     p = mlp.he_params(seed).astype(np.float64)
     x = rng.uniform(0, 1, (n, mlp.N_IN))
-    t = rng.integers(0, mlp.N_LOGITS, n)
+    if sets:
+        t = rng.random((n, mlp.N_LOGITS)) < 0.05
+        t[:, 7] = True
+    else:
+        t = rng.integers(0, mlp.N_LOGITS, n)
     eps = rng.integers(0, 256, (n, mlp.N_SEEDS))
     return p, x, t, eps
 
 
+@pytest.mark.parametrize("sets", [False, True])
 @pytest.mark.parametrize("block", ["W1", "b1", "W2", "b2", "W3", "b3"])
-def test_backward_matches_finite_differences(block):
-    p, x, t, eps = _fd_setup()
+def test_backward_matches_finite_differences(block, sets):
+    p, x, t, eps = _fd_setup(sets=sets)
     sw = 0.7
     _, g = mlp.loss_grad(p, x, t, eps, sw, exact=True)
     offs = {"W1": (mlp.OFF_W1, mlp.OFF_B1), "b1": (mlp.OFF_B1, mlp.OFF_W2),
@@ -94,3 +99,47 @@ def test_adam_first_step_is_signed_lr():
     w1, m, v = mlp.adam_step(w, g, np.zeros(100), np.zeros(100), 1, 0.01, eps=0.0)
     assert np.allclose(w1, w - 0.01 * np.sign(g))
     assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)
+
+
+def test_target_set_encodings_agree():
+    rng = np.random.default_rng(8)
+    n = 5
+    sets = rng.random((n, 256)) < 0.1
+    sets[:, 3] = True
+    words = np.zeros((n, 8), np.uint32)
+    for r in range(n):
+        for L in np.flatnonzero(sets[r]):
+            words[r, L // 32] |= np.uint32(1) << np.uint32(L % 32)
+    q1, q2 = mlp.target_sets(sets, n), mlp.target_sets(words, n)
+    assert np.array_equal(q1, q2) and np.array_equal(q1, sets)
+    # one index is the one-hot set
+    idx = rng.integers(0, 256, n)
+    one = np.zeros((n, 256), bool)
+    one[np.arange(n), idx] = True
+    assert np.array_equal(mlp.target_sets(idx, n), mlp.target_sets(one, n))
+
+
+def test_set_loss_closed_form():
+    """Zero weights: uniform softmax, so a set of m offsets costs log(256 / m);
+    an empty set costs nothing and gives no logit gradient."""
+    p = np.zeros(mlp.N_PARAMS)
+    sets = np.zeros((4, 256), bool)
+    sets[0, :5] = True
+    sets[1, 100] = True
+    sets[2, ::2] = True
+    loss, g = mlp.loss_grad(p, np.zeros((4, 48), np.uint8), sets, np.zeros((4, 6)), 1.0)
+    exp = np.mean([np.log(256 / 5), np.log(256.0), np.log(2.0), 0.0])
+    assert abs(loss - exp) < 1e-12
+    # W3 logit columns: the gradient flows only through rows with a set
+    assert np.abs(g[mlp.OFF_B3:mlp.OFF_B3 + 256]).sum() > 0
+
+
+def test_set_loss_is_cross_entropy_for_one_offset():
+    rng = np.random.default_rng(4)
+    p = mlp.he_params(4)
+    px = rng.integers(0, 256, (7, 48)).astype(np.uint8)
+    t = rng.integers(0, 256, 7)
+    Y = mlp.forward(p, px)["Y"][:, :256]
+    ce = np.mean(np.log(np.exp(Y).sum(axis=1)) - Y[np.arange(7), t])
+    loss, _ = mlp.loss_grad(p, px, t, np.zeros((7, 6)), 0.0)
+    assert abs(loss - ce) < 1e-9
