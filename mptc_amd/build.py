@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmptc_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "mptc_hip.hip")]
-HEADERS = [os.path.join(HERE, "csrc", f) for f in ("dxt_device.h", "batch_kernels.h", "framegen.h")] + [
+HEADERS = [os.path.join(HERE, "csrc", f) for f in ("dxt_device.h", "batch_kernels.h", "framegen.h",
+                                                     "mlp_kernels.h", "net_infer.h")] + [
     os.path.join(ROOT, "include", "mptc.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -25,6 +26,38 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
+
+
+HOST_DIR = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HOST_DIR, "libmptc_host.so")
+HOST_BIN = os.path.join(HOST_DIR, "encode_seq")
+CXX = os.environ.get("CXX", "g++")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"{cmd[0]} failed")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ host layer (mptc_amd/host) over libmptc_hip.so, and its
+    sequence driver; both find libmptc_hip.so through their run path."""
+    inc = os.path.join(ROOT, "include")
+    tmp = f"{HOST_LIB}.tmp{os.getpid()}"
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", inc,
+          os.path.join(HOST_DIR, "mptc_host.cpp"), "-L", HERE, "-lmptc_hip",
+          "-Wl,-rpath,$ORIGIN/..", "-o", tmp], verbose)
+    os.replace(tmp, HOST_LIB)
+    tmpb = f"{HOST_BIN}.tmp{os.getpid()}"
+    _run([CXX, "-O2", "-std=c++17", "-Wall", "-I", inc, os.path.join(HOST_DIR, "encode_seq.cpp"),
+          "-L", HOST_DIR, "-lmptc_host", "-L", HERE, "-lmptc_hip",
+          "-Wl,-rpath,$ORIGIN:$ORIGIN/..", "-o", tmpb], verbose)
+    os.replace(tmpb, HOST_BIN)
+    return HOST_LIB
 
 
 def build(verbose: bool = False, extra: list[str] | None = None) -> str:
@@ -42,6 +75,7 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> str:
         if any(l.strip() for l in lines):
             sys.stderr.write("\n".join(lines) + "\n")
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
 
 

@@ -15,7 +15,7 @@
  *
  * Parity pinning: every function here is checked against fixtures produced
  * by the compiled reference (oracle/_ref), against exhaustive or closed-form
- * checks, or both; see tests/test_oracle_*.py.
+ * checks, or both (tests/test_oracle.py).
  */
 #ifndef MPTC_ORACLE_H
 #define MPTC_ORACLE_H

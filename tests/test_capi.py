@@ -3,6 +3,7 @@ and its CPU-side pieces behave; no GPU compute here."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -55,3 +56,16 @@ def test_null_arguments_are_rejected():
     assert L.mptc_fit(None, None, 4, 4, None) == lib.MPTC_EINVAL
     assert L.mptc_reencode(None, None, 4, 4, 1, None, None, None, None, None, None, None) == lib.MPTC_EINVAL
     assert L.mptc_get_stats(None, None) == lib.MPTC_EINVAL
+
+
+def test_host_layer_is_built_and_runs_without_a_gpu():
+    """The C++ host library and its driver link against libmptc_hip.so; the
+    driver starts (no device call) and rejects bad usage."""
+    import subprocess
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(here, "mptc_amd", "host", "encode_seq")
+    assert os.path.exists(os.path.join(here, "mptc_amd", "host", "libmptc_host.so"))
+    r = subprocess.run([exe, "--version"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "mptc" in r.stdout
+    r = subprocess.run([exe, "x"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
