@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(256) k_intra_apply(const uint32_t* __restrict_
 // for part 2 (plain stores: racing writers all store 1).  Reads the changed
 // list of parity `par` and clears the one of the next pass.
 __global__ void __launch_bounds__(256) k_intra_reeval(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
-                                                      int S, const uint64_t* __restrict__ own,
+                                                      int S, int thr, const uint64_t* __restrict__ own,
                                                       const uint8_t* __restrict__ need, const uint32_t* __restrict__ cw,
                                                       const float4* __restrict__ ctab, const int* __restrict__ chg_list,
                                                       uint2* __restrict__ E4, size_t nbs, uint32_t* __restrict__ dirty,
@@ -535,15 +535,19 @@ __global__ void __launch_bounds__(256) k_intra_reeval(const FrameDev* __restrict
     const int16_t ev = res.valid ? sat16(res.err) : kNoCand;
     int16_t* slot = &E[((size_t)(s >> 2) * nbs + k) * 4 + (s & 3)];
     ++n_eval;
-    // The fold needs redoing if the delta changed, and the decision a new word
-    // if this position is the one the block currently reuses.
+    // The decision depends only on the deltas that pass the threshold (for
+    // thr >= 0: before the first passing delta the others only raise the
+    // running minimum, which that delta then undercuts; after it they can
+    // never update it; for thr < 0 nothing is ever accepted).  So the fold
+    // needs redoing if a passing delta appeared, vanished or changed, and the
+    // decision a new word if this position is the one the block reuses.
     const uint64_t dc = dec[k];
     const bool is_sel =
         dec_acc(dc) && (W2 - 1 - (int)((dc >> 40) & 0xFF)) * W2 + (W2 - 1 - (int)((dc >> 32) & 0xFF)) == s;
-    if (*slot != ev || is_sel) {
-      *slot = ev;
-      dirty[k] = 1u;
-    }
+    const int16_t old = *slot;
+    if (old != ev) *slot = ev;
+    const bool passes_old = old != kNoCand && old <= thr, passes_new = ev != kNoCand && ev <= thr;
+    if ((old != ev && (passes_old || passes_new)) || is_sel) dirty[k] = 1u;
   }
   wg_count(&s_acc, n_eval, st->reeval_cands);
 }
@@ -561,26 +565,49 @@ __global__ void __launch_bounds__(256) k_intra_select(const FrameDev* __restrict
                                                       const uint2* __restrict__ E4, size_t nbs,
                                                       uint32_t* __restrict__ dirty, uint64_t* __restrict__ dec,
                                                       int* __restrict__ chg_list, DevStats* __restrict__ st, int par) {
+  // The preceding k_intra_reeval flags blocks only from the changed list of
+  // parity par ^ 1; when that list was empty there is nothing to re-fold.
+  if (st->n_changed[par ^ 1] == 0) return;
   __shared__ unsigned int s_base;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   bool changed = false;
   if (k < total && dirty[k] != 0u) {
     dirty[k] = 0;
     const int W2 = 2 * S, groups = S * S;
+    // Only deltas that pass the threshold can change the decision (see
+    // k_intra_reeval), so the fold skips groups of four that hold none.
     Fold fd;
     int row = -1;
-#pragma unroll 8
-    for (int gi = 0; gi < groups; ++gi) {
-      const uint2 gg = E4[(size_t)gi * nbs + k];
+    // the E loads of a block are independent: issue them in batches of 32
+    // (one memory round trip each) before folding
+    for (int g0 = 0; g0 < groups; g0 += 32) {
+      uint2 buf[32];
+#pragma unroll
+      for (int b = 0; b < 32; ++b)  // past the last group: absent candidates
+        buf[b] = g0 + b < groups ? E4[(size_t)(g0 + b) * nbs + k] : make_uint2(0x7FFF7FFFu, 0x7FFF7FFFu);
+#pragma unroll
+      for (int b = 0; b < 32; ++b) {
+      const int gi = g0 + b;
+      const uint2 gg = buf[b];
+      bool any = false;
 #pragma unroll
       for (int nn = 0; nn < 4; ++nn) {
+        const int e = e_at(gg, nn);
+        any = any || (e != kNoCand && e <= thr);
+      }
+      if (!any) continue;
+#pragma unroll
+      for (int nn = 0; nn < 4; ++nn) {
+        const int e = e_at(gg, nn);
+        if (e == kNoCand || e > thr) continue;
         const int pos = 4 * gi + nn;
         const int r = pos / W2;
         if (r != row) {
           row = r;
           fd.row_done = false;
         }
-        fd.step(e_at(gg, nn), pos);
+        fd.step(e, pos);
+      }
       }
     }
     const FrameDev& d = fr[sid[k]];

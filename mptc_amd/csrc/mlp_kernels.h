@@ -332,8 +332,8 @@ __global__ void __launch_bounds__(256) k_frame_net_input(const FrameDev* __restr
 // fixpoint.  A tile not settled after kMaxRounds rounds stores, for every
 // lane, the decision its current evaluations give; what is stale is redone in
 // the next pass (never-done evaluations are stored as kNotEvaluated).  Pass
-// counters: this pass adds its changed words to n_changed[par] and clears
-// n_changed[par ^ 1] for the next pass.
+// counters: this pass adds its changed words to n_changed[par] (cleared before
+// the pass) and reads the previous pass's, n_changed[par ^ 1].
 constexpr int kMaxRounds = 512;
 constexpr int kNotEvaluated = (int)0x80000000;
 
@@ -344,12 +344,15 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
                                                      uint32_t* __restrict__ cw, uint32_t* __restrict__ seen,
                                                      int* __restrict__ errs, uint64_t* __restrict__ dec,
                                                      DevStats* __restrict__ st, int par, int first) {
+  // After a pass that changed no word every evaluation is current: nothing
+  // to do.  (n_changed[par] was cleared by a launch before this pass, so no
+  // workgroup of this pass writes the counter the others read.)
+  if (!first && st->n_changed[par ^ 1] == 0) return;
   __shared__ uint8_t s_snap[512];
   __shared__ float2 s_ab[4];
   __shared__ unsigned long long s_acc;
   __shared__ unsigned int s_chg;
   load_lds_tables(s_snap, s_ab);
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->n_changed[par ^ 1] = 0;
   if (threadIdx.x == 0) s_chg = 0;
   __syncthreads();
   const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));

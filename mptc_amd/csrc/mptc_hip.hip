@@ -555,6 +555,10 @@ int proposals_q(mptc_ctx* c, hipStream_t s, int net, const uint8_t* need, int16_
 
 int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first) {
   const int S = c->cfg.search_area, thr = c->cfg.err_threshold, kk = c->top_k;
+  int rc = launch(c, s, KF_ZERO, [&] {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, &c->d_stats->n_changed[par], 1);
+  });
+  if (rc) return rc;
   return launch(c, s, KF_LPASS, [&] {
     auto kern = kk <= 4 ? k_lintra_pass<4> : kk <= 8 ? k_lintra_pass<8> : k_lintra_pass<16>;
     hipLaunchKernelGGL(kern, dim3(grid_for(c->n_ltiles * 64, 256)), dim3(256), 0, s, c->d_fr, c->n_frames,
@@ -652,7 +656,7 @@ int queue_passes(mptc_ctx* c, hipStream_t s) {
   for (int p = 0; p < np; ++p) {
     const int par = p & 1;
     if ((rc = launch(c, s, KF_REEVAL, [&] {
-           hipLaunchKernelGGL(k_intra_reeval, dim3(kReevalGrid), dim3(256), 0, s, c->d_fr, c->d_sid, S, c->d_own,
+           hipLaunchKernelGGL(k_intra_reeval, dim3(kReevalGrid), dim3(256), 0, s, c->d_fr, c->d_sid, S, thr, c->d_own,
                               c->d_need, c->d_cw, c->d_ctab, c->d_chg, c->d_E, c->nbs, c->d_dirty, c->d_orig,
                               c->d_intra, c->d_stats, par);
          })))

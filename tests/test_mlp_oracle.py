@@ -3,12 +3,16 @@ against torch's, the backward formulas against finite differences of the
 loss, Adam against its closed form after one step."""
 import numpy as np
 import pytest
-import torch
 
 from oracle import mlp
 
 
-def test_bf16_rounding_matches_torch():
+def test_bf16_rounding_matches_torch(tmp_path):
+    """torch's float32 -> bfloat16 conversion (round to nearest even) as the
+    independent check.  It runs in a child process: torch loads its own
+    HIP/HSA runtime, which must stay out of processes that run GPU tests."""
+    import subprocess
+    import sys
     rng = np.random.default_rng(0)
     x = np.concatenate([
         rng.normal(0, 1, 20000).astype(np.float32),
@@ -19,8 +23,14 @@ def test_bf16_rounding_matches_torch():
         np.array([0.0, -0.0, np.inf, -np.inf, 3.0e38, -3.0e38, 1e-45], np.float32),
     ])
     x = x[np.isfinite(x) | np.isinf(x)]
+    np.save(tmp_path / "x.npy", x)
+    code = ("import numpy as np, torch, sys; x = np.load(sys.argv[1]); "
+            "np.save(sys.argv[2], torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy())")
+    r = subprocess.run([sys.executable, "-c", code, str(tmp_path / "x.npy"), str(tmp_path / "y.npy")],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ref = np.load(tmp_path / "y.npy")
     ours = mlp.bf16(x)
-    ref = torch.from_numpy(x.copy()).to(torch.bfloat16).to(torch.float32).numpy()
     assert np.array_equal(ours.view(np.uint32), ref.view(np.uint32))
     nan = mlp.bf16(np.array([np.nan], np.float32))
     assert np.isnan(nan[0])
