@@ -85,18 +85,31 @@ __device__ __forceinline__ bf16x8_t as_bf16x8(uint4 v) {
   return r;
 }
 
-// D values i = 8*half .. +7 of a tile, plus bias, ReLU, as a bf16 B operand.
-__device__ __forceinline__ uint4 relu_pack(const f32x16_t& d, int half, const float* __restrict__ sb, int nb,
-                                           int h) {
+// D values i = 8*half .. +7 of a tile (bias included), ReLU, as a bf16 B operand.
+__device__ __forceinline__ uint4 relu_pack(const f32x16_t& d, int half) {
   uint32_t w[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int i0 = 8 * half + 2 * q, i1 = i0 + 1;
-    const int u0 = nb * 32 + 8 * (i0 >> 2) + 4 * h + (i0 & 3), u1 = nb * 32 + 8 * (i1 >> 2) + 4 * h + (i1 & 3);
-    const float v0 = fmaxf(d[i0] + sb[u0], 0.0f), v1 = fmaxf(d[i1] + sb[u1], 0.0f);
+    const float v0 = fmaxf(d[i0], 0.0f), v1 = fmaxf(d[i1], 0.0f);
     w[q] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
   }
   return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Accumulator tile nb of a lane (half h) starting at its outputs' biases:
+// D value i is output nb * 32 + 8 (i / 4) + 4 h + i % 4.
+__device__ __forceinline__ f32x16_t bias_tile(const float* __restrict__ sb, int nb, int h) {
+  f32x16_t a;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 b4 = *reinterpret_cast<const float4*>(&sb[nb * 32 + 8 * q + 4 * h]);
+    a[4 * q] = b4.x;
+    a[4 * q + 1] = b4.y;
+    a[4 * q + 2] = b4.z;
+    a[4 * q + 3] = b4.w;
+  }
+  return a;
 }
 
 constexpr int kInferWaves = 8;  // waves per workgroup: two per SIMD share the LDS weights
@@ -108,7 +121,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l12(const uint16_t* __
                                                  const uint16_t* __restrict__ pk, uint4* __restrict__ H2p) {
   __shared__ uint4 sW1[kNetHid * kNetIn / 8];   // 24 KiB, rows of 6 chunks
   __shared__ uint4 sW2[kNetHid * kNetHid / 8];  // 128 KiB, rows of 32 chunks, swizzled
-  __shared__ float sb[2 * kNetHid];
+  __shared__ __attribute__((aligned(16))) float sb[2 * kNetHid];
   stage_rows(sW1, reinterpret_cast<const uint4*>(pk + kPkW1), kNetHid, kNetIn / 8, false);
   stage_rows(sW2, reinterpret_cast<const uint4*>(pk + kPkW2), kNetHid, kNetHid / 8, true);
   const float* bias = reinterpret_cast<const float*>(pk + kPkBias);
@@ -130,29 +143,29 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l12(const uint16_t* __
     bf16x8_t hb[16];
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb) {
-      f32x16_t acc = {};
+      f32x16_t acc = bias_tile(sb, nb, h);
 #pragma unroll
       for (int s = 0; s < 3; ++s) {
         const bf16x8_t a = as_bf16x8(sW1[(nb * 32 + m) * 6 + 2 * s + h]);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xb[s], acc, 0, 0, 0);
       }
-      hb[2 * nb] = as_bf16x8(relu_pack(acc, 0, sb, nb, h));
-      hb[2 * nb + 1] = as_bf16x8(relu_pack(acc, 1, sb, nb, h));
+      hb[2 * nb] = as_bf16x8(relu_pack(acc, 0));
+      hb[2 * nb + 1] = as_bf16x8(relu_pack(acc, 1));
       __builtin_amdgcn_sched_barrier(0);  // one accumulator live at a time
     }
     // layer 2 (one output tile at a time keeps the live registers to h1 + one accumulator)
     uint4* out = H2p + (size_t)tile * (kH2TileBytes / 16);
 #pragma unroll 1
     for (int nb = 0; nb < 8; ++nb) {
-      f32x16_t acc = {};
+      f32x16_t acc = bias_tile(sb + kNetHid, nb, h);
       const int r = nb * 32 + m;
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
         const bf16x8_t a = as_bf16x8(sW2[r * 32 + ((2 * s + h) ^ (r & 7))]);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, hb[s], acc, 0, 0, 0);
       }
-      out[(2 * nb) * 64 + lane] = relu_pack(acc, 0, sb + kNetHid, nb, h);
-      out[(2 * nb + 1) * 64 + lane] = relu_pack(acc, 1, sb + kNetHid, nb, h);
+      out[(2 * nb) * 64 + lane] = relu_pack(acc, 0);
+      out[(2 * nb + 1) * 64 + lane] = relu_pack(acc, 1);
     }
   }
 }
@@ -190,15 +203,65 @@ __device__ __forceinline__ WinMask win_mask(int net, int S, int bx, int by, int 
   return m;
 }
 
-// Rank key of an admitted offset's score: the fp32 score mapped to an
-// order-preserving unsigned integer, its low 8 bits replaced by 255 - L.  A
-// larger key ranks first; scores equal to 24 bits rank by the lower index; a
-// NaN score ranks below every number; 0 means no offset.
+// Rank key of an admitted offset's score: the fp32 score (a NaN counts as
+// -FLT_MAX) mapped to an order-preserving unsigned integer, its low 8 bits
+// replaced by 255 - L.  A larger key ranks first; scores equal to 24 bits rank
+// by the lower index; 0 means no offset.  Branch-free.
 __device__ __forceinline__ uint32_t rank_key(float v, int L) {
-  uint32_t u = __float_as_uint(v);
-  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-  if (v != v) u = 0x100u;
-  return (u & 0xFFFFFF00u) | (uint32_t)(255 - L);
+  const uint32_t u = __float_as_uint(fmaxf(v, -3.402823466e38f));
+  const uint32_t m = (uint32_t)((int32_t)u >> 31);
+  return ((u ^ (m | 0x80000000u)) & 0xFFFFFF00u) | (uint32_t)(255 - L);
+}
+
+// Bit L of the 256-bit set of admitted offsets, as words: bit b of word w is
+// offset 32 w + b (rows 2w and 2w + 1 of the 16x16 grid).
+__device__ __forceinline__ void admitted_words(const WinMask& m, uint32_t (&w)[8]) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int r0 = 2 * q, r1 = 2 * q + 1;
+    const uint32_t c0 = ((m.rows >> r0) & 1u) ? (r0 == 0 ? m.cols0 : m.cols) : 0u;
+    const uint32_t c1 = ((m.rows >> r1) & 1u) ? m.cols : 0u;
+    w[q] = (c0 & 0xFFFFu) | ((c1 & 0xFFFFu) << 16);
+  }
+}
+
+// Sorts K keys in descending order (bitonic network, branch-free).
+template <int K>
+__device__ __forceinline__ void bitonic_desc(uint32_t (&a)[K]) {
+#pragma unroll
+  for (int k = 2; k <= K; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j >= 1; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint32_t x = a[i], y = a[l];
+          const bool down = (i & k) == 0;
+          a[i] = down ? max(x, y) : min(x, y);
+          a[l] = down ? min(x, y) : max(x, y);
+        }
+      }
+}
+
+// top := the K largest of top and b, both sorted in descending order: the
+// elementwise maximum of top and reversed b is bitonic and holds them; a
+// bitonic merge sorts it.
+template <int K>
+__device__ __forceinline__ void merge_top(uint32_t (&top)[K], const uint32_t (&b)[K]) {
+#pragma unroll
+  for (int i = 0; i < K; ++i) top[i] = max(top[i], b[K - 1 - i]);
+#pragma unroll
+  for (int j = K >> 1; j >= 1; j >>= 1)
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int l = i ^ j;
+      if (l > i) {
+        const uint32_t x = top[i], y = top[l];
+        top[i] = max(x, y);
+        top[l] = min(x, y);
+      }
+    }
 }
 
 __device__ __forceinline__ int key_offset(uint32_t key) { return key ? 255 - (int)(key & 0xFFu) : -1; }
@@ -227,7 +290,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
                                                              const uint8_t* __restrict__ need,
                                                              int16_t* __restrict__ prop) {
   __shared__ uint4 sW3[kL3Rows * kNetHid / 8];  // 144 KiB, rows of 32 chunks, swizzled
-  __shared__ float sb[kL3Rows];
+  __shared__ __attribute__((aligned(16))) float sb[kL3Rows];
   stage_rows(sW3, reinterpret_cast<const uint4*>(pk + kPkW3), kL3Rows, kNetHid / 8, true);
   const float* bias = reinterpret_cast<const float*>(pk + kPkBias) + 2 * kNetHid;
   for (int i = threadIdx.x; i < kL3Rows; i += blockDim.x) sb[i] = bias[i];
@@ -237,9 +300,10 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
   const int wave_g = blockIdx.x * kInferWaves + (threadIdx.x >> 6);
   for (int tile = wave_g; tile < n_tiles; tile += gridDim.x * kInferWaves) {
     const int row = tile * 32 + m;
+    // accumulators start at the biases of the lane's outputs
     f32x16_t acc[9];
 #pragma unroll
-    for (int nb = 0; nb < 9; ++nb) acc[nb] = f32x16_t{};
+    for (int nb = 0; nb < 9; ++nb) acc[nb] = bias_tile(sb, nb, h);
     const uint4* in = H2p + (size_t)tile * (kH2TileBytes / 16);
 #pragma unroll 2
     for (int s = 0; s < 16; ++s) {
@@ -258,40 +322,48 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
 #pragma unroll
           for (int i = 0; i < 16; ++i) {
             const int u = nb * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
-            if (u < kNetOut) Y[(size_t)row * kNetOut + u] = acc[nb][i] + sb[u];
+            if (u < kNetOut) Y[(size_t)row * kNetOut + u] = acc[nb][i];
           }
       }
       continue;
     }
-    // fused top-k: each lane ranks its 128 logits; lane h = 0 merges the
-    // partner lane's (l + 32) list and writes
+    // fused top-k: each lane ranks its 128 logits in chunks of K (sorted by
+    // a bitonic network, merged into the running top K); lane h = 0 merges
+    // the partner lane's (l + 32) list and writes
     bool act = row < rows;
-    WinMask ok{0u, 0u, 0u};
+    uint32_t vw[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
     if (act) {
       const FrameDev& d = fr[sid[row]];
       const int k = row - d.base;
       act = need ? need[row] != 0 : !d.is_intra;
-      if (act) ok = win_mask(net, S, k % d.bw, k / d.bw, d.bw, d.bh);
+      if (act) admitted_words(win_mask(net, S, k % d.bw, k / d.bw, d.bw, d.bh), vw);
     }
-    uint32_t t[K];
+    uint32_t top[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i) t[i] = 0u;
+    for (int i = 0; i < K; ++i) top[i] = 0u;
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb)
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int L = nb * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
-        key_insert<K>(t, ok(L) ? rank_key(acc[nb][i] + sb[L], L) : 0u);
-      }
+      for (int c0 = 0; c0 < 16; c0 += K) {
+        uint32_t chunk[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const uint32_t pk2 = __shfl_xor(t[i], 32);
-      if (h == 0) key_insert<K>(t, pk2);
-    }
+        for (int t = 0; t < K; ++t) {
+          const int i = c0 + t;
+          const int bit = 8 * (i >> 2) + 4 * h + (i & 3);  // L = nb * 32 + bit
+          const uint32_t ok = (uint32_t)((int32_t)(vw[nb] << (31 - bit)) >> 31);
+          chunk[t] = rank_key(acc[nb][i], nb * 32 + bit) & ok;
+        }
+        bitonic_desc<K>(chunk);
+        merge_top<K>(top, chunk);
+      }
+    uint32_t partner[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) partner[i] = __shfl_xor(top[i], 32);
+    merge_top<K>(top, partner);
     if (h == 0 && act) {
 #pragma unroll
       for (int i = 0; i < K; ++i)
-        if (i < kk) prop[(size_t)row * kk + i] = (int16_t)key_offset(t[i]);
+        if (i < kk) prop[(size_t)row * kk + i] = (int16_t)key_offset(top[i]);
     }
   }
 }

@@ -282,11 +282,11 @@ def test_trained_network_beats_untrained_and_fixed_proposals(env, orc):
 
 
 def rank_keys(y, L):
-    """The proposal ranking: fp32 scores as order-preserving integers with the
-    low 8 bits replaced by 255 - L (larger first)."""
-    u = np.asarray(y, np.float32).view(np.uint32).astype(np.int64)
+    """The proposal ranking: fp32 scores (NaN as -FLT_MAX) as order-preserving
+    integers with the low 8 bits replaced by 255 - L (larger first)."""
+    y = np.where(np.isnan(y), np.float32(-3.402823466e38), np.asarray(y, np.float32)).astype(np.float32)
+    u = y.view(np.uint32).astype(np.int64)
     u = np.where(u & 0x80000000, (~u) & 0xFFFFFFFF, u | 0x80000000)
-    u = np.where(np.isnan(y), 0x100, u)
     return (u & 0xFFFFFF00) | (255 - np.asarray(L, np.int64))
 
 
