@@ -37,6 +37,10 @@ WORKLOAD = "c2_t0"
 # no FMA), ~28 solve, ~96 quantise + snap, ~24 palette, ~10 validity, ~154
 # integer SSE.
 W_CAND = 600
+# Work of one candidate bound of the pruned search: 36 v_dot4 group sums, 9
+# subtractions for the last group, 20 integer multiply/adds for |S_g|^2, 11
+# fp32 convert/multiply/adds for sum |S_g|^2 / n_g, subtract and compare.
+W_BOUND = 78
 # MI355X vector ALU: 157.3 TFLOP/s fp32 = 256 CU x 128 lanes (packed / dual
 # issue) x 2 (FMA) x 2.4 GHz, i.e. 78.6 T lane-ops/s of non-FMA fp32 work.
 VALU_PEAK = 78.6
@@ -53,6 +57,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--cpu-frames", type=int, default=2)
     ap.add_argument("--top-k", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=3000)
@@ -86,7 +91,27 @@ def cpu_baseline(spec, frames, n_frames):
 
 KERNELS = ("fit", "table", "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine", "intra_apply",
            "intra_reeval", "intra_select", "commit", "unique", "zero", "net_input", "net_l12", "net_l3",
-           "propose_inter", "top_intra", "lintra_pass")
+           "propose_inter", "top_intra", "lintra_pass", "bound_table", "pruned_inter", "fold_inter", "pruned_intra",
+           "fold_intra")
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "encode_1080p_plain_hbm_traffic.md")
+KERNEL_SYMBOL = {"inter": "k_inter", "intra_dense": "k_intra_dense", "intra_reeval": "k_intra_reeval",
+                 "pruned_inter": "k_pruned_search<false>", "pruned_intra": "k_pruned_search<true>"}
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (read + write), from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a copy
+    of known size); None when the summary has no row for it."""
+    sym = KERNEL_SYMBOL.get(kernel)
+    if sym is None or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == sym:
+            return round((float(cells[3]) + float(cells[4])) * 1e6)
+    return None
 
 
 class Runner:
@@ -123,6 +148,17 @@ class Runner:
                 stats.append(self.enc.stats())
             set_intra, num = self.lib.next_intra(set_intra, num, self.spec["intra_interval"])
         return stats
+
+    def autotune(self, steps=3):
+        """Picks the launch shape (mptc_autotune) on frame 1 of the sequence, an
+        inter frame encoded against frame 0; before any timed work."""
+        self.run(1)
+        fb = self.W * self.H * 3
+        io = dict(d_rgb=self.d_frames.offset(fb), width=self.W, height=self.H, is_intra=False,
+                  d_ref=self.d_phys[0], d_phys=self.d_phys[1 % self.n_out], d_motion=self.d_motion[0],
+                  d_unique=self.d_unique[0], d_n_unique=self.d_nuniq)
+        best, ms = self.enc.autotune([io], steps=steps)
+        return best, ms
 
     def timed(self, steps, warmup):
         self.run(warmup)
@@ -166,6 +202,8 @@ def work_of(stats):
     return {"inter_candidates": sum(s["inter_candidates"] for s in stats),
             "intra_candidates": sum(s["intra_candidates"] for s in stats),
             "reeval_candidates": sum(s["reeval_candidates"] for s in stats),
+            "exact_inter_candidates": sum(s["exact_inter_candidates"] for s in stats),
+            "exact_intra_candidates": sum(s["exact_intra_candidates"] for s in stats),
             "n_inter": sum(s["n_inter"] for s in stats), "n_intra": sum(s["n_intra"] for s in stats),
             "n_unique": sum(s["n_unique"] for s in stats),
             "max_intra_iterations": max(s["intra_iterations"] for s in stats) if stats else 0}
@@ -249,24 +287,38 @@ def main():
     nb = (W // 4) * (H // 4)
     frames = lib.synth_frames(synth_cfg(spec), 0, spec["frames"])
     enc = lib.Encoder(W, H, spec["S"], spec["thr"])
-    R = Runner(hip, lib, enc, spec, frames, max(a.steps, a.warmup, 1))
+    R = Runner(hip, lib, enc, spec, frames, max(a.steps, a.warmup, 2))
 
-    # plain (exhaustive, bit-exact) path
+    # plain (exhaustive, bit-exact) path, in the launch shape the autotuner picks
+    tuning, tune_ms = R.autotune() if not a.no_autotune else (enc.get_tuning(), None)
     elapsed = R.timed(a.steps, a.warmup)
     value = a.steps * nb / elapsed / 1e6
     kernels, ms_ev = R.kernels(a.steps)
     work = work_of(R.run(a.steps, collect=True))
     psnr_plain = R.quality(a.steps)
 
-    # roofline of the dominant search kernel (vector-ALU bound: no contraction maps to MFMA)
-    cand_of = {"inter": work["inter_candidates"], "intra_dense": work["intra_candidates"],
-               "intra_reeval": work["reeval_candidates"]}
-    dom = max((k for k in kernels if k in cand_of), key=lambda k: kernels[k]["total_ms"])
-    per_launch = cand_of[dom] / kernels[dom]["launches"]
-    achieved = per_launch * W_CAND / (kernels[dom]["avg_ms"] * 1e-3) / 1e12
+    # roofline of the dominant search kernel (vector-ALU bound: no contraction maps to MFMA).
+    # Algorithmic lane-ops of each search kernel over the run: exact evaluations
+    # x W_CAND; the pruned kernels bound every candidate and evaluate the survivors.
+    ev, bd = W_CAND, W_BOUND
+    work_of_kernel = {
+        "inter": (work["inter_candidates"] * ev, [(work["inter_candidates"], "evaluations", ev)]),
+        "intra_dense": (work["intra_candidates"] * ev, [(work["intra_candidates"], "evaluations", ev)]),
+        "intra_reeval": (work["reeval_candidates"] * ev, [(work["reeval_candidates"], "evaluations", ev)]),
+        "pruned_inter": (work["inter_candidates"] * bd + work["exact_inter_candidates"] * ev,
+                         [(work["inter_candidates"], "bounds", bd), (work["exact_inter_candidates"], "evaluations", ev)]),
+        "pruned_intra": (work["intra_candidates"] * bd + work["exact_intra_candidates"] * ev,
+                         [(work["intra_candidates"], "bounds", bd), (work["exact_intra_candidates"], "evaluations", ev)]),
+    }
+    dom = max((k for k in kernels if k in work_of_kernel), key=lambda k: kernels[k]["total_ms"])
+    lane_ops, parts = work_of_kernel[dom]
+    n_launch = kernels[dom]["launches"]
+    achieved = lane_ops / n_launch / (kernels[dom]["avg_ms"] * 1e-3) / 1e12
     roof = {"kernel": dom, "bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK,
-            "unit": "T lane-op/s", "frac": round(achieved / VALU_PEAK, 4), "traffic": None,
-            "work_per_launch": f"{per_launch:.4g} candidate evaluations x {W_CAND} lane-ops"}
+            "unit": "T lane-op/s", "frac": round(achieved / VALU_PEAK, 4), "traffic": profiled_traffic(dom),
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
+            "work_per_launch": " + ".join(f"{n / n_launch:.4g} candidate {what} x {w} lane-ops"
+                                          for n, what, w in parts)}
 
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "Mblocks/s", "n_gpus": 1,
@@ -276,7 +328,8 @@ def main():
         "config": {"workload": f"1080p encode ({a.workload}): {spec['frames']} frames {W}x{H} RGB8, "
                                f"S={spec['S']} ({2 * spec['S']}-block window), threshold {spec['thr']}, "
                                f"intra every {spec['intra_interval']} frames; value = plain search, "
-                               f"bit-exact to the reference", "blocks_per_frame": nb},
+                               f"bit-exact to the reference", "blocks_per_frame": nb,
+                   "tuning": tuning},
         "roofline": roof,
         "kernels": kernels,
         "ms_per_step_with_kernel_events": round(ms_ev, 4),

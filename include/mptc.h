@@ -58,6 +58,9 @@ typedef struct {
   int32_t use_graph;  /* replay the step as captured HIP graphs */
   int32_t win_chunks; /* window tile: the 2S window rows of a block are split into
                          this many runs scanned by separate waves, 1..8 */
+  int32_t prune;      /* 1: for S <= 8 and thresholds 0..4, evaluate exactly only the
+                         candidates an error lower bound cannot rule out (same output);
+                         0: evaluate every candidate */
 } mptc_tuning;
 
 /* One frame of a batched step: device pointers, layouts as mptc_reencode. */
@@ -85,6 +88,8 @@ typedef struct {
   int64_t reeval_candidates;/* intra positions re-evaluated after a word changed */
   int32_t intra_iterations; /* passes until the intra words reached their fixpoint */
   int32_t is_intra;
+  int64_t exact_inter_candidates; /* with pruning: candidates evaluated exactly (the rest */
+  int64_t exact_intra_candidates; /* were ruled out by the error bound); 0 without pruning */
 } mptc_frame_stats;
 
 /* Context lifetime.  Replaces the process-global state the reference keeps in
@@ -136,9 +141,10 @@ int32_t mptc_encode_frame_device(mptc_ctx* ctx, const uint8_t* d_rgb, int32_t wi
 int32_t mptc_encode_batch_device(mptc_ctx* ctx, int32_t n_frames, const mptc_frame_io* frames,
                                  void* hip_stream);
 
-/* Launch-shape tuning.  mptc_autotune times every candidate shape on `steps`
- * encodes of the given batch (outputs are overwritten), keeps the fastest and
- * reports it. */
+/* Launch-shape tuning.  mptc_autotune times every candidate shape (wg_size x
+ * tile_w x win_chunks, and prune 0/1 where the pruned search applies) on
+ * `steps` encodes of the given batch (outputs are overwritten), keeps the
+ * fastest and reports it. */
 int32_t mptc_get_tuning(mptc_ctx* ctx, mptc_tuning* out);
 int32_t mptc_set_tuning(mptc_ctx* ctx, const mptc_tuning* t);
 int32_t mptc_autotune(mptc_ctx* ctx, int32_t n_frames, const mptc_frame_io* frames, int32_t steps,
@@ -158,7 +164,8 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
  * "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine",
  * "intra_apply", "intra_reeval", "intra_select", "commit", "unique", "zero";
  * learned search: "net_input", "net_l12", "net_l3" (inference), "propose_inter",
- * "top_intra", "lintra_pass"; "net_forward": training products) since it was enabled
+ * "top_intra", "lintra_pass"; "net_forward": training products; pruned search:
+ * "bound_table", "pruned_inter", "fold_inter", "pruned_intra", "fold_intra") since it was enabled
  * (synchronises).  While enabled, steps run as individual launches, not graphs. */
 int32_t mptc_profile_enable(mptc_ctx* ctx, int32_t enable);
 int32_t mptc_profile_read(mptc_ctx* ctx, const char* name, int64_t* launches, double* total_ms);

@@ -60,6 +60,7 @@ constexpr int kStatSlots = 32;
 
 struct DevStats {
   unsigned long long inter_cands[kStatSlots], intra_cands[kStatSlots], reeval_cands[kStatSlots];
+  unsigned long long exact_inter[kStatSlots], exact_intra[kStatSlots];  // pruned search: exact evaluations
   unsigned int n_inter[kStatSlots], n_intra[kStatSlots];
   unsigned int n_unique, active_passes;  // passes that changed a word
   unsigned int n_changed[2];             // length of the changed-word list, by pass parity

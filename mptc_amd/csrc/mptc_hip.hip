@@ -25,6 +25,7 @@
 
 #include "../../include/mptc.h"
 #include "net_infer.h"
+#include "prune_kernels.h"
 #include "framegen.h"
 
 using namespace mptc;
@@ -103,13 +104,15 @@ int init_tables(int device) {
 enum KernelId {
   KF_ZERO, KF_FIT, KF_TABLE, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
   KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
-  KF_NET_L3, KF_N
+  KF_NET_L3, KF_BOUND_TABLE, KF_PRUNED_INTER, KF_FOLD_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
 };
 const char* kKernelNames[KF_N] = {"zero",         "fit",          "table",       "inter",
                                   "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
                                   "intra_apply",  "intra_reeval", "intra_select", "commit",
                                   "unique",       "net_input",   "net_forward", "propose_inter",
-                                  "top_intra",    "lintra_pass",  "net_l12",     "net_l3"};
+                                  "top_intra",    "lintra_pass",  "net_l12",     "net_l3",
+                                  "bound_table",  "pruned_inter", "fold_inter",  "pruned_intra",
+                                  "fold_intra"};
 constexpr int kMaxChunks = 8;  // window-row chunks per block in the search kernels
 
 constexpr int kReevalGrid = 4096;  // workgroups of the re-evaluation pass (grid-stride)
@@ -161,7 +164,7 @@ struct mptc_ctx {
   int max_frames = 1;
   size_t nbs = 0;     // E row stride (blocks)
   hipStream_t stream = nullptr;
-  mptc_tuning tune{256, 64, 2, 1, 2};
+  mptc_tuning tune{256, 16, 2, 1, 4, 1};  // the fastest 1080p shape measured on MI355X
   // device buffers (one entry per block of the batch unless noted)
   FrameDev* d_fr = nullptr;  // kMaxFrames descriptors
   FrameDev* h_fr = nullptr;  // pinned staging ring: 4 x kMaxFrames
@@ -184,7 +187,7 @@ struct mptc_ctx {
   uint32_t *d_unique = nullptr, *d_nunique = nullptr;
   // batch shape the maps and graphs were built for
   std::vector<int> shape;
-  mptc_tuning shape_tune{0, 0, 0, 0, 0};
+  mptc_tuning shape_tune{0, 0, 0, 0, 0, 0};
   int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0, n_ltiles = 0;
   Graphs graphs;
   int last_passes = 0, last_total = 0, last_intra = 0;
@@ -197,6 +200,9 @@ struct mptc_ctx {
   uint4* d_h2p = nullptr;     // layer-2 activations of the step (net_infer.h tiles)
   int16_t* d_pinter = nullptr;  // inter proposals, kMaxTopK per block
   int16_t* d_prop = nullptr;    // intra proposals, kMaxTopK per block
+  // pruned exhaustive search (prune_kernels.h), allocated on first use
+  uint2* d_Ei = nullptr;        // inter error deltas, layout of d_E
+  BoundTab *d_rbt = nullptr, *d_cbt = nullptr;  // bound tables of reference / current words
   bool record = false;          // exhaustive steps record acceptance masks
   bool last_recorded = false;   // the last step did
   uint16_t* d_amask[2] = {nullptr, nullptr};  // inter, intra: 16 rows of 16 offset bits per block
@@ -263,6 +269,7 @@ int check_tuning(const mptc_tuning& t) {
     return fail(MPTC_EINVAL, "tile_w must be 64, 32, 16 or 8");
   if (t.pass_batch < 1 || t.pass_batch > 16) return fail(MPTC_EINVAL, "pass_batch must be in 1..16");
   if (t.win_chunks < 1 || t.win_chunks > kMaxChunks) return fail(MPTC_EINVAL, "win_chunks must be in 1..8");
+  if (t.prune != 0 && t.prune != 1) return fail(MPTC_EINVAL, "prune must be 0 or 1");
   return MPTC_OK;
 }
 
@@ -513,6 +520,25 @@ int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
   return MPTC_OK;
 }
 
+// The pruned exhaustive search applies to small windows and thresholds, where
+// the bound rules out most candidates (thresholds 0..kPruneMaxThr).
+constexpr int kPruneMaxThr = 4;
+
+bool use_prune(const mptc_ctx* c) {
+  return c->tune.prune && c->mode == MPTC_SEARCH_EXHAUSTIVE && !c->record && c->cfg.search_area <= kPruneMaxS &&
+         c->cfg.err_threshold >= 0 && c->cfg.err_threshold <= kPruneMaxThr;
+}
+
+int prune_ensure(mptc_ctx* c) {
+  if (c->d_Ei) return MPTC_OK;
+  const size_t nb = (size_t)c->max_total;
+  const size_t s = (size_t)std::min(c->alloc_S, kPruneMaxS);  // the pruned search runs for S <= kPruneMaxS
+  int rc;
+  if ((rc = dalloc(&c->d_Ei, s * s * c->nbs)) || (rc = dalloc(&c->d_rbt, nb)) || (rc = dalloc(&c->d_cbt, nb)))
+    return rc;
+  return MPTC_OK;
+}
+
 // Fused inference over w.X -> H2 tiles (d_h2p) -> either Y (row-major) or
 // the top-kk admitted offsets of each block into prop.
 int infer_q(mptc_ctx* c, hipStream_t s, const NetState& n, const uint16_t* X, int rows, uint4* h2p, float* Y,
@@ -592,7 +618,28 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
     return rc;
   const int K = chunks_for(S, c->tune.win_chunks);
   const unsigned gwk = grid_for(c->n_wtiles * K * 64, wg);
-  if (c->mode == MPTC_SEARCH_EXHAUSTIVE) {
+  const bool prune = use_prune(c);
+  const unsigned gwk256 = grid_for(c->n_wtiles * K * 64, 256);
+  if (prune) {
+    // inter: bound, exact evaluation of the survivors, fold (prune_kernels.h)
+    if ((rc = launch(c, s, KF_BOUND_TABLE, [&] {
+           hipLaunchKernelGGL(k_bound_table_ref, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
+                              total, c->d_rw, c->d_rbt);
+         })))
+      return rc;
+    if ((rc = launch(c, s, KF_PRUNED_INTER, [&] {
+           hipLaunchKernelGGL(k_pruned_search<false>, dim3(gwk256), dim3(256), 0, s, c->d_fr, c->d_tsid, c->n_wtiles,
+                              tw, K, S, thr, c->d_own, c->d_need, c->d_rw, c->d_rtab, c->d_rbt, c->d_orig, c->d_Ei,
+                              c->nbs, c->d_stats);
+         })))
+      return rc;
+    if ((rc = launch(c, s, KF_FOLD_INTER, [&] {
+           hipLaunchKernelGGL(k_fold_search<false>, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
+                              total, S, thr, c->d_own, c->d_need, c->d_rw, c->d_Ei, c->nbs, c->d_inter, c->d_cwn,
+                              c->d_chg, c->d_stats);
+         })))
+      return rc;
+  } else if (c->mode == MPTC_SEARCH_EXHAUSTIVE) {
     if ((rc = launch(c, s, KF_INTER, [&] {
            hipLaunchKernelGGL(k_inter, dim3(gwk), dim3(wg), 0, s, c->d_fr, c->d_tsid, c->n_wtiles, tw, K, S,
                               c->d_own, c->d_rw, c->d_rtab, c->d_summ, c->nbs, c->d_stats, thr,
@@ -620,6 +667,29 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
   if (c->mode == MPTC_SEARCH_LEARNED) {
     if ((rc = proposals_q(c, s, MPTC_NET_INTRA, c->d_need, c->d_prop, x_done))) return rc;
     return lintra_pass_q(c, s, 0, 1);
+  }
+  if (prune) {
+    if ((rc = launch(c, s, KF_BOUND_TABLE, [&] {
+           hipLaunchKernelGGL(k_bound_table_cur, dim3(grid_for(total, 256)), dim3(256), 0, s, total, c->d_cw,
+                              c->d_cbt);
+         })))
+      return rc;
+    if ((rc = launch(c, s, KF_PRUNED_INTRA, [&] {
+           hipLaunchKernelGGL(k_pruned_search<true>, dim3(gwk256), dim3(256), 0, s, c->d_fr, c->d_tsid, c->n_wtiles,
+                              tw, K, S, thr, c->d_own, c->d_need, c->d_cw, c->d_ctab, c->d_cbt, c->d_orig, c->d_E,
+                              c->nbs, c->d_stats);
+         })))
+      return rc;
+    if ((rc = launch(c, s, KF_FOLD_INTRA, [&] {
+           hipLaunchKernelGGL(k_fold_search<true>, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
+                              total, S, thr, c->d_own, c->d_need, c->d_cw, c->d_E, c->nbs, c->d_intra, c->d_cwn,
+                              c->d_chg, c->d_stats);
+         })))
+      return rc;
+    return launch(c, s, KF_APPLY, [&] {
+      hipLaunchKernelGGL(k_intra_apply, dim3(kListGrid), dim3(256), 0, s, c->d_cwn, c->d_cw, c->d_ctab, c->d_chg,
+                         c->d_stats);
+    });
   }
   if ((rc = launch(c, s, KF_DENSE, [&] {
          hipLaunchKernelGGL(k_intra_dense, dim3(gwk), dim3(wg), 0, s, c->d_fr, c->d_tsid, c->n_wtiles, tw, K, S,
@@ -765,6 +835,7 @@ int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
 int encode_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in, bool fit) {
   int rc = prepare_batch(c, s, n, in);
   if (rc) return rc;
+  if (use_prune(c) && (rc = prune_ensure(c))) return rc;
   c->last_intra = n == 1 ? in[0].is_intra : 0;
   return run_step(c, s, fit);
 }
@@ -859,7 +930,8 @@ void mptc_destroy(mptc_ctx* c) {
   net_free(c->net[1]);
   c->nw.release();
   c->ew.release();
-  void* lbufs[] = {c->d_h2p, c->d_pinter, c->d_prop, c->d_seen, c->d_errs, c->d_amask[0], c->d_amask[1]};
+  void* lbufs[] = {c->d_h2p, c->d_pinter, c->d_prop, c->d_seen, c->d_errs, c->d_amask[0], c->d_amask[1],
+                   c->d_Ei,  c->d_rbt,    c->d_cbt};
   for (void* p : lbufs)
     if (p) (void)hipFree(p);
   if (c->h_stats) (void)hipHostFree(c->h_stats);
@@ -978,10 +1050,16 @@ int32_t mptc_autotune(mptc_ctx* c, int32_t n, const mptc_frame_io* frames, int32
   const int wgs[] = {64, 128, 256};
   const int tws[] = {64, 32, 16, 8};
   const int chunks[] = {1, 2, 4, 8};
+  // the pruned search is tried where it applies (both give the same output)
+  mptc_tuning probe = keep;
+  probe.prune = 1;
+  c->tune = probe;
+  const int n_prune = use_prune(c) ? 2 : 1;
+  for (int pr = 0; pr < n_prune; ++pr)
   for (int wg : wgs)
     for (int tw : tws)
       for (int kc : chunks) {
-        c->tune = mptc_tuning{wg, tw, keep.pass_batch, keep.use_graph, kc};
+        c->tune = mptc_tuning{wg, tw, keep.pass_batch, keep.use_graph, kc, n_prune == 2 ? pr : keep.prune};
         int rc = mptc_encode_batch_device(c, n, frames, nullptr);  // warm-up: maps and graphs
         if (rc) {
           c->tune = keep;
@@ -1023,12 +1101,15 @@ int32_t mptc_get_stats(mptc_ctx* c, mptc_frame_stats* out) {
   HIP_TRY(hipMemcpy(&d, c->d_stats, sizeof(d), hipMemcpyDeviceToHost));
   out->n_blocks = c->last_total;
   out->n_inter = out->n_intra = out->inter_candidates = out->intra_candidates = out->reeval_candidates = 0;
+  out->exact_inter_candidates = out->exact_intra_candidates = 0;
   for (int i = 0; i < kStatSlots; ++i) {
     out->n_inter += d.n_inter[i];
     out->n_intra += d.n_intra[i];
     out->inter_candidates += (int64_t)d.inter_cands[i];
     out->intra_candidates += (int64_t)d.intra_cands[i];
     out->reeval_candidates += (int64_t)d.reeval_cands[i];
+    out->exact_inter_candidates += (int64_t)d.exact_inter[i];
+    out->exact_intra_candidates += (int64_t)d.exact_intra[i];
   }
   out->n_unique = d.n_unique;
   out->intra_iterations = (int32_t)d.active_passes + 1;

@@ -1,0 +1,358 @@
+// Pruned exhaustive search: the same decisions as k_inter / k_intra_dense
+// (bit-exact), with most candidate evaluations replaced by a cheap lower
+// bound on their error.
+//
+// Why it is exact.  For a threshold thr >= 0 a block's decision depends only
+// on the candidates whose error delta passes (valid, delta <= thr): before the
+// first passing candidate (scan order) the others can only raise the running
+// minimum, which that candidate then undercuts; after it they can never update
+// it (codec/dxt_image.cpp:697-708).  So a candidate proved non-passing may be
+// recorded as any non-passing value (kPruned) without changing the decision.
+//
+// The bound.  Any palette gives one colour per index value, so the squared
+// error of a block under index word c is at least the scatter of its pixels
+// around their group means, W_c = sum |p|^2 - sum_g |S_g|^2 / n_g (S_g, n_g:
+// sum and count of the pixels with index g).  Error() = floor(SSE / 48)
+// (:258-276), so delta >= floor(W_c / 48) - err_own, and the candidate cannot
+// pass when W_c >= 48 (thr + err_own + 1).  W_c is computed from integer group
+// sums (v_dot4 over byte masks of the word) and one fp32 division per group,
+// with a 64 margin covering the fp32 rounding (< 48).
+//
+//   k_pruned_search wave tiles as the search kernels; per position the bound;
+//                   non-passing -> kPruned in E; the wave's survivors are
+//                   compacted in LDS and evaluated exactly, 64 per round
+//   k_fold_search   per block, the reference's selection over the passing
+//                   deltas of E: the inter decision, or the intra decision
+//                   with the changed-word list (as k_intra_combine)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "batch_kernels.h"
+
+namespace mptc {
+
+constexpr int16_t kPruned = 32766;  // E value of a candidate proved not to pass
+constexpr int kPruneMaxS = 8;       // window positions of one batch fit the LDS list
+
+// Per index word: byte masks of groups 0..2 over the four 4-pixel chunks and
+// 1 / (pixels per group) for groups 0..3 (0 for an empty group).
+struct BoundTab {
+  uint4 m0, m1, m2;
+  float4 inv;
+};
+
+__device__ __forceinline__ BoundTab bound_table(uint32_t c) {
+  uint32_t m[3][4];
+  uint32_t n[3] = {0u, 0u, 0u};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t x = spread_indices(c, j);
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      m[g][j] = byte_gather(1u << (8 * g), x);  // 1 in byte n where pixel 4j+n has index g
+      n[g] = __builtin_amdgcn_udot4(m[g][j], 0x01010101u, n[g], false);
+    }
+  }
+  const uint32_t n3 = 16u - n[0] - n[1] - n[2];
+  BoundTab t;
+  t.m0 = make_uint4(m[0][0], m[0][1], m[0][2], m[0][3]);
+  t.m1 = make_uint4(m[1][0], m[1][1], m[1][2], m[1][3]);
+  t.m2 = make_uint4(m[2][0], m[2][1], m[2][2], m[2][3]);
+  t.inv = make_float4(n[0] ? 1.0f / (float)n[0] : 0.0f, n[1] ? 1.0f / (float)n[1] : 0.0f,
+                      n[2] ? 1.0f / (float)n[2] : 0.0f, n3 ? 1.0f / (float)n3 : 0.0f);
+  return t;
+}
+
+// Pixel side of the bound for one block.
+struct BoundBlk {
+  uint32_t pc[12];  // pc[3j + ch]: channel ch of pixels 4j..4j+3
+  int tot[3];       // channel sums
+  uint32_t sq;      // sum of squared channel values
+  float q;          // the same as fp32 (exact)
+  float lim;        // prune when W_c >= lim = 48 (thr + err_own + 1) + 64
+};
+
+// Prune when W_c >= 48 (thr + err_own + 1) + 64 (64: the fp32 rounding margin).
+__device__ __forceinline__ float bound_limit(int thr, int err_own) { return (float)(48 * (thr + err_own + 1) + 64); }
+
+__device__ __forceinline__ void make_bound_blk(const BlockPx& px, int err_own, int thr, BoundBlk& b) {
+  uint32_t sq = 0;
+  int tot[3] = {0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      const uint32_t v = px.ch(4 * j, ch) | (px.ch(4 * j + 1, ch) << 8) | (px.ch(4 * j + 2, ch) << 16) |
+                         (px.ch(4 * j + 3, ch) << 24);
+      b.pc[3 * j + ch] = v;
+      sq = __builtin_amdgcn_udot4(v, v, sq, false);
+      tot[ch] = (int)__builtin_amdgcn_udot4(v, 0x01010101u, (uint32_t)tot[ch], false);
+    }
+  b.tot[0] = tot[0];
+  b.tot[1] = tot[1];
+  b.tot[2] = tot[2];
+  b.sq = sq;
+  b.q = (float)sq;  // <= 16 * 3 * 255^2 < 2^24: exact
+  b.lim = bound_limit(thr, err_own);
+}
+
+__device__ __forceinline__ int group_sum(const uint4& m, const BoundBlk& b, int ch) {
+  uint32_t s = __builtin_amdgcn_udot4(m.x, b.pc[ch], 0u, false);
+  s = __builtin_amdgcn_udot4(m.y, b.pc[3 + ch], s, false);
+  s = __builtin_amdgcn_udot4(m.z, b.pc[6 + ch], s, false);
+  s = __builtin_amdgcn_udot4(m.w, b.pc[9 + ch], s, false);
+  return (int)s;
+}
+
+// True when the candidate word of `t` provably does not pass the threshold.
+__device__ __forceinline__ bool bound_prunes(const BoundBlk& b, const BoundTab& t) {
+  int s0[3], s1[3], s2[3], s3[3];
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    s0[ch] = group_sum(t.m0, b, ch);
+    s1[ch] = group_sum(t.m1, b, ch);
+    s2[ch] = group_sum(t.m2, b, ch);
+    s3[ch] = b.tot[ch] - s0[ch] - s1[ch] - s2[ch];
+  }
+  // 0 <= S_g <= 4080 (12 bits): 24-bit multiplies are exact; |S_g|^2 <= 3 * 4080^2 < 2^26 fits
+  // int32, and the fp32 products and sum err by < 48 in total
+  auto sq3 = [](const int* v) {
+    return (int)(__umul24((uint32_t)v[0], (uint32_t)v[0]) + __umul24((uint32_t)v[1], (uint32_t)v[1]) +
+                 __umul24((uint32_t)v[2], (uint32_t)v[2]));
+  };
+  const int q0 = sq3(s0), q1 = sq3(s1), q2 = sq3(s2), q3 = sq3(s3);
+  const float T = (float)q0 * t.inv.x + (float)q1 * t.inv.y + (float)q2 * t.inv.z + (float)q3 * t.inv.w;
+  return b.q - T >= b.lim;
+}
+
+// Window position pos of block (bx, by) -> the candidate block (j, i); false
+// outside the frame or (intra) outside the causal window.
+template <bool kIntra>
+__device__ __forceinline__ bool window_block(int pos, int S, int bx, int by, int bw, int bh, int& j, int& i) {
+  const int W2 = 2 * S, r = pos / W2, t = pos % W2;
+  if (kIntra) {
+    j = by - r;
+    i = bx + S - 1 - t;
+    return j >= 0 && i >= 0 && i < bw && !(r == 0 && t < S);
+  }
+  j = by - S + r;
+  i = bx - S + t;
+  return j >= 0 && j < bh && i >= 0 && i < bw;
+}
+
+constexpr int kPruneBatch = 16;  // window positions per bound / evaluate round of a wave
+constexpr int kPackedBlk = 15;   // LDS words per block: pc[12], sum of squares, own word, own error
+
+// Pruned search over the window of every searching block (inter: blocks of
+// inter frames against the previous frame's words; intra: blocks with need[]
+// set against the current words), in the wave-tile mapping of the dense
+// kernels.  Per batch of kPruneBatch positions each lane bounds its block's
+// candidates; the survivors of the whole wave are compacted in LDS and
+// evaluated exactly 64 at a time, a lane taking any block's survivor (the
+// pixel side of the wave's 64 blocks is kept in LDS).  E gets the exact delta
+// of survivors, kPruned for pruned and kNoCand for absent positions.
+template <bool kIntra>
+__global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ tsid,
+                                                       int n_wtiles, int tw, int n_chunks, int S, int thr,
+                                                       const uint64_t* __restrict__ own,
+                                                       const uint8_t* __restrict__ need,
+                                                       const uint32_t* __restrict__ words,
+                                                       const float4* __restrict__ tabs,
+                                                       const BoundTab* __restrict__ btab, const int* __restrict__ orig,
+                                                       uint2* __restrict__ E4, size_t nbs, DevStats* __restrict__ st) {
+  __shared__ uint8_t s_snap[512];
+  __shared__ float2 s_ab[4];
+  __shared__ uint32_t s_blk[4][kPackedBlk][64];      // [wave][field][lane]
+  __shared__ int16_t s_e[4][kPruneBatch][64];        // the batch's E values, [wave][position][lane]
+  __shared__ uint16_t s_list[4][kPruneBatch * 64];   // survivors: lane | position << 6
+  __shared__ unsigned long long s_acc;
+  load_lds_tables(s_snap, s_ab);
+  int f = 0, bx = 0, by = 0, chunk = 0;
+  const bool in = tile_block(fr, tsid, n_wtiles, tw, n_chunks, f, bx, by, chunk);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const FrameDev& d = fr[f];
+  const int g = in ? d.base + by * d.bw + bx : 0;
+  const bool act = in && (kIntra ? need[g] != 0 : !d.is_intra);
+  // origin of the wave tile (a wave with survivors is a tile inside the batch)
+  const int bx0 = bx - lane % tw, by0 = by - lane / tw;
+  BoundBlk b;
+  if (act) {
+    BlockPx px;
+    load_block(d.rgb, d.width, bx, by, px);
+    const int e_own = kIntra ? orig[g] : own_error(px, own[g]);
+    make_bound_blk(px, e_own, thr, b);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s_blk[wave][k][lane] = b.pc[k];
+    s_blk[wave][12][lane] = b.sq;
+    s_blk[wave][13][lane] = (uint32_t)(own[g] >> 32);
+    s_blk[wave][14][lane] = (uint32_t)e_own;
+  }
+  unsigned long long n_bound = 0, n_exact = 0;
+  const int W2 = 2 * S, p0 = chunk * (W2 / n_chunks) * W2, p1 = p0 + (W2 / n_chunks) * W2;
+  for (int pb = p0; pb < p1; pb += kPruneBatch) {
+    const int nb = min(kPruneBatch, p1 - pb);  // a multiple of 4
+    uint32_t cnt = 0;                          // wave-uniform
+    for (int k = 0; k < nb; ++k) {
+      int j = 0, i = 0;
+      const bool there = act && window_block<kIntra>(pb + k, S, bx, by, d.bw, d.bh, j, i);
+      int ev = kNoCand;
+      bool surv = false;
+      if (there) {
+        ++n_bound;
+        if (bound_prunes(b, btab[d.base + j * d.bw + i]))
+          ev = kPruned;
+        else
+          surv = true;
+      }
+      const unsigned long long bal = __ballot(surv);
+      if (surv)
+        s_list[wave][cnt + (uint32_t)__popcll(bal & ((1ull << lane) - 1))] = (uint16_t)(lane | (k << 6));
+      else
+        s_e[wave][k][lane] = (int16_t)ev;
+      cnt += (uint32_t)__popcll(bal);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
+      if (e0 + lane < cnt) {
+        const uint32_t e = s_list[wave][e0 + lane];
+        const int lb = (int)(e & 63u), k = (int)(e >> 6);
+        int j = 0, i = 0;
+        window_block<kIntra>(pb + k, S, bx0 + lb % tw, by0 + lb / tw, d.bw, d.bh, j, i);
+        const int q = d.base + j * d.bw + i;
+        BlockEval be;
+#pragma unroll
+        for (int w = 0; w < 12; ++w) be.pc[w] = s_blk[wave][w][lb];
+#pragma unroll
+        for (int p = 0; p < 16; ++p)
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) be.f[3 * p + ch] = (float)((be.pc[3 * (p >> 2) + ch] >> (8 * (p & 3))) & 0xFFu);
+        be.sq = s_blk[wave][12][lb];
+        be.own = s_blk[wave][13][lb];
+        be.orig = (int)s_blk[wave][14][lb];
+        const float4 t4 = tabs[q];
+        const CandResult res = eval_candidate<false>(be, words[q], CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap, s_ab);
+        s_e[wave][k][lb] = res.valid ? sat16(res.err) : kNoCand;
+        ++n_exact;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (act)
+      for (int k = 0; k < nb; k += 4) {
+        const uint32_t lo = (uint32_t)(uint16_t)s_e[wave][k][lane] | ((uint32_t)(uint16_t)s_e[wave][k + 1][lane] << 16);
+        const uint32_t hi =
+            (uint32_t)(uint16_t)s_e[wave][k + 2][lane] | ((uint32_t)(uint16_t)s_e[wave][k + 3][lane] << 16);
+        E4[(size_t)((pb + k) >> 2) * nbs + g] = make_uint2(lo, hi);
+      }
+    // the next batch overwrites s_e and s_list
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  wg_count(&s_acc, n_bound, kIntra ? st->intra_cands : st->inter_cands);
+  wg_count(&s_acc, n_exact, kIntra ? st->exact_intra : st->exact_inter);
+}
+
+// The selection rule (codec/dxt_image.cpp:697-708) over the passing deltas of
+// block g's E (for thr >= 0 that is the full rule's result; see above).  The
+// S^2 groups of four are loaded 32 at a time.
+__device__ __forceinline__ Fold fold_passing(const uint2* __restrict__ E4, size_t nbs, int g, int S, int thr) {
+  const int W2 = 2 * S, groups = S * S;
+  Fold fd;
+  int row = -1;
+  for (int g0 = 0; g0 < groups; g0 += 32) {
+    uint2 buf[32];
+#pragma unroll
+    for (int b = 0; b < 32; ++b)  // past the last group: absent candidates
+      buf[b] = g0 + b < groups ? E4[(size_t)(g0 + b) * nbs + g] : make_uint2(0x7FFF7FFFu, 0x7FFF7FFFu);
+#pragma unroll
+    for (int b = 0; b < 32; ++b) {
+      const uint2 gg = buf[b];
+      bool any = false;
+#pragma unroll
+      for (int nn = 0; nn < 4; ++nn) {
+        const int e = e_at(gg, nn);
+        any = any || (e != kNoCand && e <= thr);
+      }
+      if (!any) continue;
+#pragma unroll
+      for (int nn = 0; nn < 4; ++nn) {
+        const int e = e_at(gg, nn);
+        if (e == kNoCand || e > thr) continue;
+        const int pos = 4 * (g0 + b) + nn;
+        const int r = pos / W2;
+        if (r != row) {
+          row = r;
+          fd.row_done = false;
+        }
+        fd.step(e, pos);
+      }
+    }
+  }
+  return fd;
+}
+
+// Decisions from E.  Inter: the accept test (:890) -> dec (as k_inter_combine).
+// Intra: as k_intra_combine -> dec, the new current word and the changed list.
+template <bool kIntra>
+__global__ void __launch_bounds__(256) k_fold_search(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
+                                                     int total, int S, int thr, const uint64_t* __restrict__ own,
+                                                     const uint8_t* __restrict__ need,
+                                                     const uint32_t* __restrict__ words,
+                                                     const uint2* __restrict__ E4, size_t nbs,
+                                                     uint64_t* __restrict__ dec, uint32_t* __restrict__ cw_new,
+                                                     int* __restrict__ chg_list, DevStats* __restrict__ st) {
+  __shared__ unsigned int s_base;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  bool changed = false;
+  if (g < total) {
+    const FrameDev& d = fr[sid[g]];
+    const int k = g - d.base, bx = k % d.bw, by = k / d.bw;
+    if (kIntra) {
+      if (need[g]) {
+        const Fold fd = fold_passing(E4, nbs, g, S, thr);
+        const uint64_t dc = intra_decision(words, d.base, d.bw, bx, by, S, thr, fd);
+        dec[g] = dc;
+        const uint32_t w = dec_acc(dc) ? dec_word(dc) : (uint32_t)(own[g] >> 32);
+        if (w != words[g]) {
+          cw_new[g] = w;
+          changed = true;
+        }
+      }
+    } else if (!d.is_intra) {
+      const Fold fd = fold_passing(E4, nbs, g, S, thr);
+      uint64_t dc = 0;
+      if (fd.sel >= 0 && fd.min_err <= thr) {
+        const int W2 = 2 * S, r = fd.sel / W2, t = fd.sel % W2;
+        dc = pack_dec(words[d.base + (by - S + r) * d.bw + (bx - S + t)], (uint32_t)t | 0x80u,
+                      (uint32_t)r | 0x80u, true);
+      }
+      dec[g] = dc;
+    }
+  }
+  if (kIntra) wg_append(changed, g, chg_list, &st->n_changed[0], &s_base);
+}
+
+// Bound tables of the previous frames' words (inter candidates).
+__global__ void __launch_bounds__(256) k_bound_table_ref(const FrameDev* __restrict__ fr,
+                                                         const uint8_t* __restrict__ sid, int total,
+                                                         const uint32_t* __restrict__ rw,
+                                                         BoundTab* __restrict__ rbt) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total || fr[sid[g]].is_intra) return;
+  rbt[g] = bound_table(rw[g]);
+}
+
+// Bound tables of the current words at the start of the intra search.
+__global__ void __launch_bounds__(256) k_bound_table_cur(int total, const uint32_t* __restrict__ cw,
+                                                         BoundTab* __restrict__ cbt) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < total) cbt[g] = bound_table(cw[g]);
+}
+
+}  // namespace mptc

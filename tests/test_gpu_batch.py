@@ -80,9 +80,10 @@ def test_batch_of_uneven_streams_matches_reference(orc):
     enc.close()
 
 
-@pytest.mark.parametrize("wg,tw,pb,graph,chunks", [(256, 64, 2, 1, 2), (64, 8, 1, 0, 1), (128, 16, 2, 1, 4),
-                                                   (256, 32, 3, 0, 8), (64, 64, 16, 1, 6), (256, 16, 1, 1, 3)])
-def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks):
+@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune", [(256, 64, 2, 1, 2, 1), (64, 8, 1, 0, 1, 1),
+                                                         (128, 16, 2, 1, 4, 0), (256, 32, 3, 0, 8, 1),
+                                                         (64, 64, 16, 1, 6, 0), (256, 16, 1, 1, 3, 1)])
+def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks, prune):
     from mptc_amd import hip, lib
     from oracle import SynthCfg
     sizes = [(160, 96, 11), (96, 64, 12), (256, 128, 13)]
@@ -91,10 +92,16 @@ def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks):
     want = [orc.encode_sequence(fr, S, thr, 3) for fr in frames]
     total = sum((w // 4) * (h // 4) for w, h, _ in sizes)
     enc = lib.Encoder(256, 128, S, thr, max_batch_frames=3, max_batch_blocks=total)
-    enc.set_tuning(wg, tw, pb, graph, chunks)
+    enc.set_tuning(wg, tw, pb, graph, chunks, prune)
     streams = [Stream(hip, fr) for fr in frames]
-    log(f"tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks}")
+    log(f"tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune}")
     res = _encode_streams(enc, streams, 4, 3)
+    st = enc.stats()
+    exact = st["exact_inter_candidates"] + st["exact_intra_candidates"]
+    if prune:  # S = 6, threshold 2: the bound rules out candidates
+        assert 0 < exact < st["inter_candidates"] + st["intra_candidates"]
+    else:
+        assert exact == 0
     for f in range(4):
         for i in range(3):
             compare_results(res[f][i], want[i][f], f"stream {i} frame {f}")

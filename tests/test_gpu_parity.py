@@ -137,6 +137,19 @@ def test_negative_and_large_thresholds(enc, orc):
         _oracle_vs_gpu(enc, orc, frames, 4, thr, 4, f"threshold {thr}")
 
 
+def test_pruned_search_thresholds(enc, orc):
+    """Thresholds 0..4 with S <= 8 take the pruned search (bound + exact
+    evaluation of the survivors); it must match the oracle bit for bit."""
+    from oracle import SynthCfg
+    frames = orc.synth_frames(SynthCfg(160, 96, 91, 5, 1, 3, 8, 50), 5)
+    for S in (1, 3, 8):
+        for thr in (0, 2, 4):
+            _oracle_vs_gpu(enc, orc, frames, S, thr, 3, f"S {S} threshold {thr}")
+            st = enc.stats()
+            assert st["exact_inter_candidates"] <= st["inter_candidates"]
+            assert st["exact_intra_candidates"] <= st["intra_candidates"]
+
+
 def test_constant_and_two_colour_frames(enc, orc):
     h, w = 32, 48
     const = np.full((3, h, w, 3), 97, np.uint8)
