@@ -142,7 +142,7 @@ int32_t mptc_encode_batch_device(mptc_ctx* ctx, int32_t n_frames, const mptc_fra
                                  void* hip_stream);
 
 /* Launch-shape tuning.  mptc_autotune times every candidate shape (wg_size x
- * tile_w x win_chunks, and prune 0/1 where the pruned search applies) on
+ * tile_w x win_chunks x use_graph, and prune 0/1 where the pruned search applies) on
  * `steps` encodes of the given batch (outputs are overwritten), keeps the
  * fastest and reports it. */
 int32_t mptc_get_tuning(mptc_ctx* ctx, mptc_tuning* out);
@@ -160,9 +160,10 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
 
 /* Per-kernel timing with HIP events recorded on the launch stream.
  * mptc_profile_enable(ctx, 1) starts recording; mptc_profile_read returns the
- * accumulated launch count and milliseconds of kernel `name` ("fit", "table",
+ * accumulated launch count and milliseconds of kernel `name` ("fit": the step's
+ * per-block preparation (fit, reference tables, network input), "zero",
  * "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine",
- * "intra_apply", "intra_reeval", "intra_select", "commit", "unique", "zero";
+ * "intra_apply", "intra_reeval", "intra_select", "commit", "unique";
  * learned search: "net_input", "net_l12", "net_l3" (inference), "propose_inter",
  * "top_intra", "lintra_pass"; "net_forward": training products; pruned search:
  * "bound_table", "pruned_inter", "fold_inter", "pruned_intra", "fold_intra") since it was enabled

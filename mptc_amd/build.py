@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmptc_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "mptc_hip.hip")]
 HEADERS = [os.path.join(HERE, "csrc", f) for f in ("dxt_device.h", "batch_kernels.h", "framegen.h",
-                                                     "mlp_kernels.h", "net_infer.h")] + [
+                                                     "mlp_kernels.h", "net_infer.h", "prune_kernels.h")] + [
     os.path.join(ROOT, "include", "mptc.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -4,8 +4,9 @@
 //
 //   k_fill_sid     block -> frame index
 //   k_zero_u32     counters (a kernel, so that the step can be graph-captured)
-//   k_fit          stb DXT1 fit of every block (lane per block)
-//   k_table_ref    candidate table of the reference frames' index words
+//   k_fit          stb DXT1 fit of every block (lane per block); in the step
+//                  k_block_prep (mlp_kernels.h) does it together with the
+//                  candidate tables of the reference frames' index words
 //   k_inter        InterBlockSearch + accept test (lane per block, wave tile)
 //   k_intra_prep   which blocks need the intra search; initial current words
 //   k_intra_dense  every intra candidate of every such block: stores the error
@@ -64,6 +65,7 @@ struct DevStats {
   unsigned int n_inter[kStatSlots], n_intra[kStatSlots];
   unsigned int n_unique, active_passes;  // passes that changed a word
   unsigned int n_changed[2];             // length of the changed-word list, by pass parity
+  unsigned int pass_done;                // workgroups of the running fixpoint pass that finished
 };
 
 // Per-block search decision, packed into 64 bits:
@@ -88,6 +90,35 @@ __device__ __forceinline__ void wg_count(unsigned long long* s_acc, unsigned lon
   if (v) atomicAdd(s_acc, v);
   __syncthreads();
   if (threadIdx.x == 0 && *s_acc) atomicAdd(&slots[blockIdx.x % kStatSlots], *s_acc);
+}
+
+// End of a fixpoint pass's workgroup (thread 0 acts; call it from every
+// workgroup exactly once, after the workgroup's counter updates).  The last
+// workgroup to finish knows that every workgroup has read its inputs and added
+// its changes: it clears `clear` (the counter the next pass adds to, or null),
+// rearms pass_done and writes the pass's count n_changed[par] to `conv`
+// (pinned host memory the host polls for convergence; null: not written).
+// The workgroups' counter updates are atomics whose results they consume
+// before they get here, so they are complete before their pass_done add (no
+// device-scope fence, which would write back the XCD's L2).
+__device__ __forceinline__ void pass_end(DevStats* st, int par, unsigned int* clear, unsigned int* conv) {
+  if (threadIdx.x != 0) return;
+  const unsigned t = atomicAdd(&st->pass_done, 1u);
+  if (t + 1 != gridDim.x) return;
+  const unsigned n = atomicAdd(&st->n_changed[par], 0u);
+  if (clear) atomicExch(clear, 0u);
+  atomicExch(&st->pass_done, 0u);
+  if (conv) {
+    *reinterpret_cast<volatile unsigned int*>(conv) = n;
+    __threadfence_system();
+  }
+}
+
+// Clears two u32 ranges (one launch for the step's counters).
+__global__ void __launch_bounds__(256) k_zero2_u32(uint32_t* __restrict__ a, int na, uint32_t* __restrict__ b, int nb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < na) a[i] = 0u;
+  else if (i - na < nb) b[i - na] = 0u;
 }
 
 // Appends v (where want) to a global list with one atomic per workgroup.
@@ -221,19 +252,6 @@ __global__ void __launch_bounds__(256) k_fit(const FrameDev* __restrict__ fr, co
   BlockPx px;
   load_block(d.rgb, d.width, k % d.bw, k / d.bw, px);
   own[g] = fit_block(px);
-}
-
-// Candidate table of the reference frames' index words.
-__global__ void __launch_bounds__(256) k_table_ref(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid, int total,
-                            uint32_t* __restrict__ rw, float4* __restrict__ rtab) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const FrameDev& d = fr[sid[g]];
-  if (d.is_intra) return;
-  const uint32_t c = (uint32_t)(d.ref[g - d.base] >> 32);
-  const CandTab t = cand_table(c);
-  rw[g] = c;
-  rtab[g] = make_float4(t.asq, t.bsq, t.ab, t.f);
 }
 
 // InterBlockSearch (codec/dxt_image.cpp:715-774): window rows j = by-S+r
@@ -559,16 +577,21 @@ __global__ void __launch_bounds__(256) k_intra_reeval(const FrameDev* __restrict
 // word in this pass may see the old or the new value: either way the next
 // pass re-evaluates every pair that sees the word, so the passes still end at
 // the same fixpoint.  Writes the changed list of parity `par` (cleared by the
-// preceding k_intra_reeval).
+// preceding k_intra_reeval); its length goes to *conv when the pass ends.
 __global__ void __launch_bounds__(256) k_intra_select(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
                                                       int total, int S, int thr, const uint64_t* __restrict__ own,
                                                       uint32_t* __restrict__ cw, float4* __restrict__ ctab,
                                                       const uint2* __restrict__ E4, size_t nbs,
                                                       uint32_t* __restrict__ dirty, uint64_t* __restrict__ dec,
-                                                      int* __restrict__ chg_list, DevStats* __restrict__ st, int par) {
+                                                      int* __restrict__ chg_list, DevStats* __restrict__ st, int par,
+                                                      unsigned int* __restrict__ conv) {
   // The preceding k_intra_reeval flags blocks only from the changed list of
   // parity par ^ 1; when that list was empty there is nothing to re-fold.
-  if (st->n_changed[par ^ 1] == 0) return;
+  // (No workgroup of this pass writes that counter.)
+  if (st->n_changed[par ^ 1] == 0) {
+    pass_end(st, par, nullptr, conv);
+    return;
+  }
   __shared__ unsigned int s_base;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   bool changed = false;
@@ -624,6 +647,7 @@ __global__ void __launch_bounds__(256) k_intra_select(const FrameDev* __restrict
     }
   }
   wg_append(changed, k, chg_list, &st->n_changed[par], &s_base);
+  pass_end(st, par, nullptr, conv);
 }
 
 // Which intra offsets pass the accept test at the fixpoint (the training

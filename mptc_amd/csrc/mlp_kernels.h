@@ -292,17 +292,9 @@ __global__ void __launch_bounds__(256) k_collect_masks(const FrameDev* __restric
   }
 }
 
-// Network input of every block of a batch: total x 48 bf16 (pixel / 255).
-__global__ void __launch_bounds__(256) k_frame_net_input(const FrameDev* __restrict__ fr,
-                                                         const uint8_t* __restrict__ sid, int total,
-                                                         uint16_t* __restrict__ X) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const FrameDev& d = fr[sid[g]];
-  const int k = g - d.base;
-  BlockPx b;
-  load_block(d.rgb, d.width, k % d.bw, k / d.bw, b);
-  uint4* o = reinterpret_cast<uint4*>(X + (size_t)g * kNetIn);
+// Network input row of one block: 48 bf16 (pixel bytes / 255, in memory order).
+__device__ __forceinline__ void net_input_row(const BlockPx& b, uint16_t* __restrict__ row) {
+  uint4* o = reinterpret_cast<uint4*>(row);
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     uint32_t h[4];
@@ -314,6 +306,43 @@ __global__ void __launch_bounds__(256) k_frame_net_input(const FrameDev* __restr
     }
     o[q] = make_uint4(h[0], h[1], h[2], h[3]);
   }
+}
+
+// Network input of every block of a batch: total x 48 bf16.
+__global__ void __launch_bounds__(256) k_frame_net_input(const FrameDev* __restrict__ fr,
+                                                         const uint8_t* __restrict__ sid, int total,
+                                                         uint16_t* __restrict__ X) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const FrameDev& d = fr[sid[g]];
+  const int k = g - d.base;
+  BlockPx b;
+  load_block(d.rgb, d.width, k % d.bw, k / d.bw, b);
+  net_input_row(b, X + (size_t)g * kNetIn);
+}
+
+// The step's per-block preparation in one launch: the candidate table of the
+// reference frame's index word (blocks of inter frames; as k_table_ref), and
+// from one load of the block's pixels the stb fit (own, when not null; as
+// k_fit) and the network input row (X, when not null).
+__global__ void __launch_bounds__(256) k_block_prep(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
+                                                    int total, uint64_t* __restrict__ own, uint32_t* __restrict__ rw,
+                                                    float4* __restrict__ rtab, uint16_t* __restrict__ X) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const FrameDev& d = fr[sid[g]];
+  const int k = g - d.base;
+  if (!d.is_intra) {
+    const uint32_t c = (uint32_t)(d.ref[k] >> 32);
+    const CandTab t = cand_table(c);
+    rw[g] = c;
+    rtab[g] = make_float4(t.asq, t.bsq, t.ab, t.f);
+  }
+  if (!own && !X) return;
+  BlockPx px;
+  load_block(d.rgb, d.width, k % d.bw, k / d.bw, px);
+  if (own) own[g] = fit_block(px);
+  if (X) net_input_row(px, X + (size_t)g * kNetIn);
 }
 
 // Learned intra search, one fixpoint pass.  A wave owns an 8x8 tile of
@@ -332,8 +361,9 @@ __global__ void __launch_bounds__(256) k_frame_net_input(const FrameDev* __restr
 // fixpoint.  A tile not settled after kMaxRounds rounds stores, for every
 // lane, the decision its current evaluations give; what is stale is redone in
 // the next pass (never-done evaluations are stored as kNotEvaluated).  Pass
-// counters: this pass adds its changed words to n_changed[par] (cleared before
-// the pass) and reads the previous pass's, n_changed[par ^ 1].
+// counters: this pass adds its changed words to n_changed[par] (zero at its
+// start) and reads the previous pass's, n_changed[par ^ 1], which its last
+// workgroup clears for the next pass (pass_end); the count also goes to *conv.
 constexpr int kMaxRounds = 512;
 constexpr int kNotEvaluated = (int)0x80000000;
 
@@ -343,15 +373,21 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
                                                      const uint64_t* __restrict__ own, const uint8_t* __restrict__ need,
                                                      uint32_t* __restrict__ cw, uint32_t* __restrict__ seen,
                                                      int* __restrict__ errs, uint64_t* __restrict__ dec,
-                                                     DevStats* __restrict__ st, int par, int first) {
-  // After a pass that changed no word every evaluation is current: nothing
-  // to do.  (n_changed[par] was cleared by a launch before this pass, so no
-  // workgroup of this pass writes the counter the others read.)
-  if (!first && st->n_changed[par ^ 1] == 0) return;
+                                                     DevStats* __restrict__ st, int par, int first,
+                                                     unsigned int* __restrict__ conv) {
   __shared__ uint8_t s_snap[512];
   __shared__ float2 s_ab[4];
   __shared__ unsigned long long s_acc;
-  __shared__ unsigned int s_chg;
+  __shared__ unsigned int s_chg, s_prev;
+  // After a pass that changed no word every evaluation is current: nothing
+  // to do.  The previous pass's count is read once, before this workgroup
+  // reports its end (the last workgroup to end clears it).
+  if (threadIdx.x == 0) s_prev = first ? 1u : st->n_changed[par ^ 1];
+  __syncthreads();
+  if (s_prev == 0) {
+    pass_end(st, par, &st->n_changed[par ^ 1], conv);
+    return;
+  }
   load_lds_tables(s_snap, s_ab);
   if (threadIdx.x == 0) s_chg = 0;
   __syncthreads();
@@ -487,6 +523,7 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
   }
   wg_count(&s_acc, n_cand, st->intra_cands);
   wg_count(&s_acc, n_re, st->reeval_cands);
+  pass_end(st, par, &st->n_changed[par ^ 1], conv);
 }
 
 }  // namespace mptc

@@ -102,11 +102,11 @@ int init_tables(int device) {
 }
 
 enum KernelId {
-  KF_ZERO, KF_FIT, KF_TABLE, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
+  KF_ZERO, KF_FIT, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
   KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
   KF_NET_L3, KF_BOUND_TABLE, KF_PRUNED_INTER, KF_FOLD_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
 };
-const char* kKernelNames[KF_N] = {"zero",         "fit",          "table",       "inter",
+const char* kKernelNames[KF_N] = {"zero",         "fit",          "inter",
                                   "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
                                   "intra_apply",  "intra_reeval", "intra_select", "commit",
                                   "unique",       "net_input",   "net_forward", "propose_inter",
@@ -164,7 +164,7 @@ struct mptc_ctx {
   int max_frames = 1;
   size_t nbs = 0;     // E row stride (blocks)
   hipStream_t stream = nullptr;
-  mptc_tuning tune{256, 16, 2, 1, 4, 1};  // the fastest 1080p shape measured on MI355X
+  mptc_tuning tune{256, 16, 2, 0, 4, 1};  // the fastest 1080p shape measured on MI355X
   // device buffers (one entry per block of the batch unless noted)
   FrameDev* d_fr = nullptr;  // kMaxFrames descriptors
   FrameDev* h_fr = nullptr;  // pinned staging ring: 4 x kMaxFrames
@@ -178,7 +178,8 @@ struct mptc_ctx {
   uint4* d_summ = nullptr;  // kMaxChunks selection summaries per block
   DevStats* d_stats = nullptr;
   DevStats* h_stats = nullptr;  // pinned
-  uint32_t* h_pass = nullptr;   // pinned: changed-list length after each queued batch
+  uint32_t* h_conv = nullptr;   // pinned, device-mapped: changed-word count of the last finished pass
+  uint32_t* d_conv = nullptr;   // its device address
   std::vector<hipEvent_t> batch_ev;
   // staging of the single-frame host-memory entry points
   uint8_t* d_rgb = nullptr;
@@ -581,15 +582,11 @@ int proposals_q(mptc_ctx* c, hipStream_t s, int net, const uint8_t* need, int16_
 
 int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first) {
   const int S = c->cfg.search_area, thr = c->cfg.err_threshold, kk = c->top_k;
-  int rc = launch(c, s, KF_ZERO, [&] {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, &c->d_stats->n_changed[par], 1);
-  });
-  if (rc) return rc;
   return launch(c, s, KF_LPASS, [&] {
     auto kern = kk <= 4 ? k_lintra_pass<4> : kk <= 8 ? k_lintra_pass<8> : k_lintra_pass<16>;
     hipLaunchKernelGGL(kern, dim3(grid_for(c->n_ltiles * 64, 256)), dim3(256), 0, s, c->d_fr, c->n_frames,
                        c->n_ltiles, S, thr, kk, c->d_prop, c->d_own, c->d_need, c->d_cw, c->d_seen, c->d_errs,
-                       c->d_intra, c->d_stats, par, first);
+                       c->d_intra, c->d_stats, par, first, c->d_conv);
   });
 }
 
@@ -598,22 +595,21 @@ int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first) {
 int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
   const int total = c->total, wg = c->tune.wg_size, tw = c->tune.tile_w;
   const int S = c->cfg.search_area, thr = c->cfg.err_threshold;
-  bool x_done = false;
   int rc;
   if ((rc = launch(c, s, KF_ZERO, [&] {
-         hipLaunchKernelGGL(k_zero_u32, dim3(grid_for((int)(sizeof(DevStats) / 4), 256)), dim3(256), 0, s,
-                            reinterpret_cast<uint32_t*>(c->d_stats), (int)(sizeof(DevStats) / 4));
-         hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(c->n_tiles, 256)), dim3(256), 0, s, c->d_tiles, c->n_tiles);
+         const int n_st = (int)(sizeof(DevStats) / 4);
+         hipLaunchKernelGGL(k_zero2_u32, dim3(grid_for(n_st + c->n_tiles, 256)), dim3(256), 0, s,
+                            reinterpret_cast<uint32_t*>(c->d_stats), n_st, c->d_tiles, c->n_tiles);
        })))
     return rc;
-  if (fit && (rc = launch(c, s, KF_FIT, [&] {
-                hipLaunchKernelGGL(k_fit, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
-                                   c->d_own);
-              })))
-    return rc;
-  if ((rc = launch(c, s, KF_TABLE, [&] {
-         hipLaunchKernelGGL(k_table_ref, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
-                            c->d_rw, c->d_rtab);
+  // fit, reference tables and (learned modes) the network input, in one launch
+  const bool want_x = c->mode != MPTC_SEARCH_EXHAUSTIVE && c->ew.X &&
+                      (!c->net[MPTC_NET_INTER].use_prior ||
+                       (c->mode == MPTC_SEARCH_LEARNED && !c->net[MPTC_NET_INTRA].use_prior));
+  bool x_done = want_x;
+  if ((rc = launch(c, s, KF_FIT, [&] {
+         hipLaunchKernelGGL(k_block_prep, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
+                            fit ? c->d_own : nullptr, c->d_rw, c->d_rtab, want_x ? c->ew.X : nullptr);
        })))
     return rc;
   const int K = chunks_for(S, c->tune.win_chunks);
@@ -734,7 +730,7 @@ int queue_passes(mptc_ctx* c, hipStream_t s) {
     if ((rc = launch(c, s, KF_SELECT, [&] {
            hipLaunchKernelGGL(k_intra_select, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total, S,
                               thr, c->d_own, c->d_cw, c->d_ctab, c->d_E, c->nbs, c->d_dirty, c->d_intra, c->d_chg,
-                              c->d_stats, par ^ 1);
+                              c->d_stats, par ^ 1, c->d_conv);
          })))
       return rc;
   }
@@ -806,8 +802,6 @@ int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
       if (r) return r;
     }
     passes += np;
-    // length of the changed list the batch's last pass produced (parity 0)
-    HIP_TRY(hipMemcpyAsync(&c->h_pass[slot], &c->d_stats->n_changed[0], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipEventRecord(c->batch_ev[slot], s));
     return MPTC_OK;
   };
@@ -818,7 +812,11 @@ int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
     if (passes > c->total + 4 * np) return fail(MPTC_ESTATE, "intra fixpoint did not converge");
     if ((rc = queue_batch(slot ^ 1))) return rc;
     HIP_TRY(hipEventSynchronize(c->batch_ev[slot]));
-    if (c->h_pass[slot] == 0) break;  // converged; the batch queued after it runs empty
+    // Every pass ends by writing its count of changed words to h_conv, so once
+    // the batch is done h_conv holds the count of its last pass or of a later
+    // one.  A pass that changes nothing proves the fixpoint (every later pass
+    // is empty too); the batch queued after it then runs empty.
+    if (*reinterpret_cast<volatile uint32_t*>(c->h_conv) == 0) break;
     slot ^= 1;
   }
   if (graphs) {
@@ -903,7 +901,9 @@ int32_t mptc_create(const mptc_config* cfg, mptc_ctx** out) {
       (rc = dalloc(&c->d_kind, one_nb)) || (rc = dalloc(&c->d_unique, one_nb)) || (rc = dalloc(&c->d_nunique, 1)))
     return cleanup(rc);
   if (hipHostMalloc(reinterpret_cast<void**>(&c->h_stats), sizeof(DevStats), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&c->h_pass), 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->h_conv), 64 * sizeof(uint32_t),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_conv), c->h_conv, 0) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&c->h_fr), 4 * kMaxFrames * sizeof(FrameDev), hipHostMallocDefault) !=
           hipSuccess)
     return cleanup(fail(MPTC_ENOMEM, "hipHostMalloc failed"));
@@ -935,7 +935,7 @@ void mptc_destroy(mptc_ctx* c) {
   for (void* p : lbufs)
     if (p) (void)hipFree(p);
   if (c->h_stats) (void)hipHostFree(c->h_stats);
-  if (c->h_pass) (void)hipHostFree(c->h_pass);
+  if (c->h_conv) (void)hipHostFree(c->h_conv);
   if (c->h_fr) (void)hipHostFree(c->h_fr);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1056,10 +1056,11 @@ int32_t mptc_autotune(mptc_ctx* c, int32_t n, const mptc_frame_io* frames, int32
   c->tune = probe;
   const int n_prune = use_prune(c) ? 2 : 1;
   for (int pr = 0; pr < n_prune; ++pr)
+  for (int graph = 0; graph < 2; ++graph)
   for (int wg : wgs)
     for (int tw : tws)
       for (int kc : chunks) {
-        c->tune = mptc_tuning{wg, tw, keep.pass_batch, keep.use_graph, kc, n_prune == 2 ? pr : keep.prune};
+        c->tune = mptc_tuning{wg, tw, keep.pass_batch, graph, kc, n_prune == 2 ? pr : keep.prune};
         int rc = mptc_encode_batch_device(c, n, frames, nullptr);  // warm-up: maps and graphs
         if (rc) {
           c->tune = keep;

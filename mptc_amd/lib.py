@@ -243,7 +243,7 @@ class Encoder:
         _check(self.L.mptc_get_tuning(self.h, C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, wg_size=256, tile_w=16, pass_batch=2, use_graph=1, win_chunks=4, prune=1):
+    def set_tuning(self, wg_size=256, tile_w=16, pass_batch=2, use_graph=0, win_chunks=4, prune=1):
         t = Tuning(wg_size, tile_w, pass_batch, use_graph, win_chunks, prune)
         _check(self.L.mptc_set_tuning(self.h, C.byref(t)))
 
