@@ -144,13 +144,27 @@ __device__ __forceinline__ void load_lds_tables(uint8_t* s_snap, float2* s_ab) {
   __syncthreads();
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  This renumbering gives each XCD a contiguous range of workgroup ids (a
+// bijection on 0..gridDim.x-1: XCD x takes q + (x < r) ids from base
+// x*q + min(x, r), q = n / 8, r = n % 8), so that with raster-ordered tiles
+// an XCD works on one band of the frame and the window data it reads stays in
+// its L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block_id() {
+  const int n = (int)gridDim.x, i = (int)blockIdx.x;
+  const int x = i % kXcds, s = i / kXcds, q = n / kXcds, r = n % kXcds;
+  return x * q + min(x, r) + s;
+}
+
 // Work-item mapping of the search kernels: wave w covers chunk (w % n_chunks)
 // of the window of wave tile (w / n_chunks); a wave tile is tw x (64/tw)
-// blocks of one frame.  Returns false for lanes beyond the frame edge.
+// blocks of one frame.  Workgroup ids are XCD-banded (xcd_block_id).
+// Returns false for lanes beyond the frame edge.
 __device__ __forceinline__ bool tile_block(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ tsid,
                                            int n_wtiles, int tw, int n_chunks, int& f, int& bx, int& by,
                                            int& chunk) {
-  const int wi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int wi = xcd_block_id() * blockDim.x + threadIdx.x;
   // a wave is one chunk of one tile of one frame: keep the frame index (and so
   // the frame's fields) in scalar registers
   const int w = __builtin_amdgcn_readfirstlane(wi >> 6), l = wi & 63;
@@ -160,8 +174,8 @@ __device__ __forceinline__ bool tile_block(const FrameDev* __restrict__ fr, cons
   f = __builtin_amdgcn_readfirstlane(tsid[t]);
   const int lt = t - fr[f].wtile_base;
   const int tx = lt % fr[f].tiles_x, ty = lt / fr[f].tiles_x;
-  bx = tx * tw + l % tw;
-  by = ty * (64 / tw) + l / tw;
+  bx = tx * tw + (l & (tw - 1));  // tw is a power of two
+  by = ty * (64 / tw) + (l >> __builtin_ctz((unsigned)tw));
   return bx < fr[f].bw && by < fr[f].bh;
 }
 

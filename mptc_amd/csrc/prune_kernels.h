@@ -127,11 +127,11 @@ __device__ __forceinline__ bool bound_prunes(const BoundBlk& b, const BoundTab& 
   return b.q - T >= b.lim;
 }
 
-// Window position pos of block (bx, by) -> the candidate block (j, i); false
-// outside the frame or (intra) outside the causal window.
+// Window position (row r, column t; position r * 2S + t) of block (bx, by) ->
+// the candidate block (j, i); false outside the frame or (intra) outside the
+// causal window.
 template <bool kIntra>
-__device__ __forceinline__ bool window_block(int pos, int S, int bx, int by, int bw, int bh, int& j, int& i) {
-  const int W2 = 2 * S, r = pos / W2, t = pos % W2;
+__device__ __forceinline__ bool window_block(int r, int t, int S, int bx, int by, int bw, int bh, int& j, int& i) {
   if (kIntra) {
     j = by - r;
     i = bx + S - 1 - t;
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
   __shared__ float2 s_ab[4];
   __shared__ uint32_t s_blk[4][kPackedBlk][64];      // [wave][field][lane]
   __shared__ int16_t s_e[4][kPruneBatch][64];        // the batch's E values, [wave][position][lane]
-  __shared__ uint16_t s_list[4][kPruneBatch * 64];   // survivors: lane | position << 6
+  __shared__ uint16_t s_list[4][kPruneBatch * 64 + 64];  // survivors: lane | position << 6; + per-lane dummies
   __shared__ unsigned long long s_acc;
   load_lds_tables(s_snap, s_ab);
   int f = 0, bx = 0, by = 0, chunk = 0;
@@ -175,9 +175,11 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
   const FrameDev& d = fr[f];
   const int g = in ? d.base + by * d.bw + bx : 0;
   const bool act = in && (kIntra ? need[g] != 0 : !d.is_intra);
-  // origin of the wave tile (a wave with survivors is a tile inside the batch)
-  const int bx0 = bx - lane % tw, by0 = by - lane / tw;
-  BoundBlk b;
+  // origin of the wave tile (a wave with survivors is a tile inside the batch);
+  // tw is a power of two
+  const int tw_log = __builtin_ctz((unsigned)tw);
+  const int bx0 = bx - (lane & (tw - 1)), by0 = by - (lane >> tw_log);
+  BoundBlk b{};  // zero for lanes that do not search: their bounds are computed and ignored
   if (act) {
     BlockPx px;
     load_block(d.rgb, d.width, bx, by, px);
@@ -190,27 +192,37 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
     s_blk[wave][14][lane] = (uint32_t)e_own;
   }
   unsigned long long n_bound = 0, n_exact = 0;
-  const int W2 = 2 * S, p0 = chunk * (W2 / n_chunks) * W2, p1 = p0 + (W2 / n_chunks) * W2;
-  for (int pb = p0; pb < p1; pb += kPruneBatch) {
+  const int W2 = 2 * S, rows = W2 / n_chunks, p0 = chunk * rows * W2, p1 = p0 + rows * W2;
+  // (rb, tb): window row and column of the batch's first position, stepped
+  // without divisions
+  int rb = chunk * rows, tb = 0;
+  // A wave none of whose blocks searches has nothing to do.  Otherwise every
+  // lane computes every bound (a vector instruction costs the same for any
+  // number of active lanes) and the lanes without a candidate ignore theirs:
+  // no divergent branches in the loop.
+  const bool wave_act = __ballot(act) != 0ull;
+  // the frame's fields in registers (the loops' stores could otherwise alias them)
+  const int fbw = d.bw, fbh = d.bh, fbase = d.base;
+  for (int pb = p0; wave_act && pb < p1; pb += kPruneBatch) {
     const int nb = min(kPruneBatch, p1 - pb);  // a multiple of 4
     uint32_t cnt = 0;                          // wave-uniform
+    int r = rb, t = tb;
     for (int k = 0; k < nb; ++k) {
       int j = 0, i = 0;
-      const bool there = act && window_block<kIntra>(pb + k, S, bx, by, d.bw, d.bh, j, i);
-      int ev = kNoCand;
-      bool surv = false;
-      if (there) {
-        ++n_bound;
-        if (bound_prunes(b, btab[d.base + j * d.bw + i]))
-          ev = kPruned;
-        else
-          surv = true;
-      }
+      const bool there = act & window_block<kIntra>(r, t, S, bx, by, fbw, fbh, j, i);
+      t = t + 1 == W2 ? 0 : t + 1;
+      r += t == 0 ? 1 : 0;
+      // clamped into the frame: the load is always in bounds
+      const int q = fbase + min(max(j, 0), fbh - 1) * fbw + min(max(i, 0), fbw - 1);
+      const bool pruned = bound_prunes(b, btab[q]);
+      const bool surv = there && !pruned;
+      n_bound += there ? 1u : 0u;
       const unsigned long long bal = __ballot(surv);
-      if (surv)
-        s_list[wave][cnt + (uint32_t)__popcll(bal & ((1ull << lane) - 1))] = (uint16_t)(lane | (k << 6));
-      else
-        s_e[wave][k][lane] = (int16_t)ev;
+      // survivors take the next list slots; the others write their lane's dummy slot
+      const uint32_t slot = surv ? cnt + (uint32_t)__popcll(bal & ((1ull << lane) - 1))
+                                 : (uint32_t)(kPruneBatch * 64 + lane);
+      s_list[wave][slot] = (uint16_t)(lane | (k << 6));
+      s_e[wave][k][lane] = (int16_t)(there ? kPruned : kNoCand);  // survivors' entries are replaced below
       cnt += (uint32_t)__popcll(bal);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -220,9 +232,14 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
       if (e0 + lane < cnt) {
         const uint32_t e = s_list[wave][e0 + lane];
         const int lb = (int)(e & 63u), k = (int)(e >> 6);
+        int re = rb, te = tb + k;  // k < kPruneBatch: a few steps at most
+        while (te >= W2) {
+          te -= W2;
+          ++re;
+        }
         int j = 0, i = 0;
-        window_block<kIntra>(pb + k, S, bx0 + lb % tw, by0 + lb / tw, d.bw, d.bh, j, i);
-        const int q = d.base + j * d.bw + i;
+        window_block<kIntra>(re, te, S, bx0 + (lb & (tw - 1)), by0 + (lb >> tw_log), fbw, fbh, j, i);
+        const int q = fbase + j * fbw + i;
         BlockEval be;
 #pragma unroll
         for (int w = 0; w < 12; ++w) be.pc[w] = s_blk[wave][w][lb];
@@ -253,6 +270,8 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    rb = r;
+    tb = t;
   }
   wg_count(&s_acc, n_bound, kIntra ? st->intra_cands : st->inter_cands);
   wg_count(&s_acc, n_exact, kIntra ? st->exact_intra : st->exact_inter);
