@@ -161,7 +161,7 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
 /* Per-kernel timing with HIP events recorded on the launch stream.
  * mptc_profile_enable(ctx, 1) starts recording; mptc_profile_read returns the
  * accumulated launch count and milliseconds of kernel `name` ("fit": the step's
- * per-block preparation (fit, reference tables, network input), "zero",
+ * first kernel: counters cleared, fit, reference tables, network input),
  * "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine",
  * "intra_apply", "intra_reeval", "intra_select", "commit", "unique";
  * learned search: "net_input", "net_l12", "net_l3" (inference), "propose_inter",

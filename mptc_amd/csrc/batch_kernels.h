@@ -3,7 +3,6 @@
 // the frames' blocks: global block id g = frame base + by*bw + bx.
 //
 //   k_fill_sid     block -> frame index
-//   k_zero_u32     counters (a kernel, so that the step can be graph-captured)
 //   k_fit          stb DXT1 fit of every block (lane per block); in the step
 //                  k_block_prep (mlp_kernels.h) does it together with the
 //                  candidate tables of the reference frames' index words
@@ -112,13 +111,6 @@ __device__ __forceinline__ void pass_end(DevStats* st, int par, unsigned int* cl
     *reinterpret_cast<volatile unsigned int*>(conv) = n;
     __threadfence_system();
   }
-}
-
-// Clears two u32 ranges (one launch for the step's counters).
-__global__ void __launch_bounds__(256) k_zero2_u32(uint32_t* __restrict__ a, int na, uint32_t* __restrict__ b, int nb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < na) a[i] = 0u;
-  else if (i - na < nb) b[i - na] = 0u;
 }
 
 // Appends v (where want) to a global list with one atomic per workgroup.
@@ -242,12 +234,6 @@ __device__ __forceinline__ void combine_chunks(const uint4* __restrict__ summ, s
     }
   }
   ret = m;
-}
-
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_zero_u32(uint32_t* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_fill_sid(const FrameDev* __restrict__ fr, int n_frames, uint8_t* __restrict__ sid) {
@@ -601,11 +587,10 @@ __global__ void __launch_bounds__(256) k_intra_select(const FrameDev* __restrict
                                                       unsigned int* __restrict__ conv) {
   // The preceding k_intra_reeval flags blocks only from the changed list of
   // parity par ^ 1; when that list was empty there is nothing to re-fold.
-  // (No workgroup of this pass writes that counter.)
-  if (st->n_changed[par ^ 1] == 0) {
-    pass_end(st, par, nullptr, conv);
-    return;
-  }
+  // No workgroup of this pass writes that counter, so either every workgroup
+  // returns here or none does; an empty pass reports nothing (pass_end): the
+  // pass that produced the empty list already wrote 0 to *conv.
+  if (st->n_changed[par ^ 1] == 0) return;
   __shared__ unsigned int s_base;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   bool changed = false;

@@ -321,14 +321,19 @@ __global__ void __launch_bounds__(256) k_frame_net_input(const FrameDev* __restr
   net_input_row(b, X + (size_t)g * kNetIn);
 }
 
-// The step's per-block preparation in one launch: the candidate table of the
-// reference frame's index word (blocks of inter frames; as k_table_ref), and
-// from one load of the block's pixels the stb fit (own, when not null; as
-// k_fit) and the network input row (X, when not null).
+// The step's first kernel: clears the step's counters (z1[0..n1), z2[0..n2))
+// and does the per-block preparation: the candidate table of the reference
+// frame's index word (blocks of inter frames), and from one load of the
+// block's pixels the stb fit (own, when not null; as k_fit) and the network
+// input row (X, when not null).  The grid covers max(total, n1 + n2) items.
 __global__ void __launch_bounds__(256) k_block_prep(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
                                                     int total, uint64_t* __restrict__ own, uint32_t* __restrict__ rw,
-                                                    float4* __restrict__ rtab, uint16_t* __restrict__ X) {
+                                                    float4* __restrict__ rtab, uint16_t* __restrict__ X,
+                                                    uint32_t* __restrict__ z1, int n1, uint32_t* __restrict__ z2,
+                                                    int n2) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n1) z1[g] = 0u;
+  else if (g - n1 < n2) z2[g - n1] = 0u;
   if (g >= total) return;
   const FrameDev& d = fr[sid[g]];
   const int k = g - d.base;
@@ -381,13 +386,13 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
   __shared__ unsigned int s_chg, s_prev;
   // After a pass that changed no word every evaluation is current: nothing
   // to do.  The previous pass's count is read once, before this workgroup
-  // reports its end (the last workgroup to end clears it).
+  // reports its end (the last workgroup to end clears it).  It is the same
+  // for every workgroup (only the last one to report changes it), so either
+  // all return here or none does; an empty pass reports nothing: its counter
+  // stays zero and the previous pass already wrote 0 to *conv.
   if (threadIdx.x == 0) s_prev = first ? 1u : st->n_changed[par ^ 1];
   __syncthreads();
-  if (s_prev == 0) {
-    pass_end(st, par, &st->n_changed[par ^ 1], conv);
-    return;
-  }
+  if (s_prev == 0) return;
   load_lds_tables(s_snap, s_ab);
   if (threadIdx.x == 0) s_chg = 0;
   __syncthreads();

@@ -102,11 +102,11 @@ int init_tables(int device) {
 }
 
 enum KernelId {
-  KF_ZERO, KF_FIT, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
+  KF_FIT, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
   KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
   KF_NET_L3, KF_BOUND_TABLE, KF_PRUNED_INTER, KF_FOLD_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
 };
-const char* kKernelNames[KF_N] = {"zero",         "fit",          "inter",
+const char* kKernelNames[KF_N] = {"fit",          "inter",
                                   "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
                                   "intra_apply",  "intra_reeval", "intra_select", "commit",
                                   "unique",       "net_input",   "net_forward", "propose_inter",
@@ -125,6 +125,15 @@ struct Graphs {
     if (passes) (void)hipGraphExecDestroy(passes);
     if (suffix) (void)hipGraphExecDestroy(suffix);
     prefix = passes = suffix = nullptr;
+  }
+};
+// The step's graphs for a batch with inter frames (v[1]) and for a batch of
+// intra frames only (v[0]), which skips the inter search.
+struct GraphSet {
+  Graphs v[2];
+  void destroy() {
+    v[0].destroy();
+    v[1].destroy();
   }
 };
 
@@ -190,7 +199,8 @@ struct mptc_ctx {
   std::vector<int> shape;
   mptc_tuning shape_tune{0, 0, 0, 0, 0, 0};
   int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0, n_ltiles = 0;
-  Graphs graphs;
+  GraphSet graphs;
+  bool any_inter = true;  // the batch holds an inter frame
   int last_passes = 0, last_total = 0, last_intra = 0;
   // learned proposals
   int mode = MPTC_SEARCH_EXHAUSTIVE, top_k = 8;
@@ -514,6 +524,8 @@ int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
     c->shape_tune = c->tune;
   }
   c->n_frames = n;
+  c->any_inter = false;
+  for (int f = 0; f < n; ++f) c->any_inter = c->any_inter || !in[f].is_intra;
   c->total = base;
   c->n_wtiles = wt;
   c->n_tiles = tb;
@@ -596,27 +608,26 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
   const int total = c->total, wg = c->tune.wg_size, tw = c->tune.tile_w;
   const int S = c->cfg.search_area, thr = c->cfg.err_threshold;
   int rc;
-  if ((rc = launch(c, s, KF_ZERO, [&] {
-         const int n_st = (int)(sizeof(DevStats) / 4);
-         hipLaunchKernelGGL(k_zero2_u32, dim3(grid_for(n_st + c->n_tiles, 256)), dim3(256), 0, s,
-                            reinterpret_cast<uint32_t*>(c->d_stats), n_st, c->d_tiles, c->n_tiles);
-       })))
-    return rc;
-  // fit, reference tables and (learned modes) the network input, in one launch
+  // counters, fit, reference tables and (learned modes) the network input, in one launch
   const bool want_x = c->mode != MPTC_SEARCH_EXHAUSTIVE && c->ew.X &&
                       (!c->net[MPTC_NET_INTER].use_prior ||
                        (c->mode == MPTC_SEARCH_LEARNED && !c->net[MPTC_NET_INTRA].use_prior));
   bool x_done = want_x;
   if ((rc = launch(c, s, KF_FIT, [&] {
-         hipLaunchKernelGGL(k_block_prep, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
-                            fit ? c->d_own : nullptr, c->d_rw, c->d_rtab, want_x ? c->ew.X : nullptr);
+         const int n_st = (int)(sizeof(DevStats) / 4);
+         hipLaunchKernelGGL(k_block_prep, dim3(grid_for(std::max(total, n_st + c->n_tiles), wg)), dim3(wg), 0, s,
+                            c->d_fr, c->d_sid, total, fit ? c->d_own : nullptr, c->d_rw, c->d_rtab,
+                            want_x ? c->ew.X : nullptr, reinterpret_cast<uint32_t*>(c->d_stats), n_st, c->d_tiles,
+                            c->n_tiles);
        })))
     return rc;
   const int K = chunks_for(S, c->tune.win_chunks);
   const unsigned gwk = grid_for(c->n_wtiles * K * 64, wg);
   const bool prune = use_prune(c);
   const unsigned gwk256 = grid_for(c->n_wtiles * K * 64, 256);
-  if (prune) {
+  if (!c->any_inter) {
+    // a batch of intra frames only: no inter search (k_intra_prep reads no inter decision)
+  } else if (prune) {
     // inter: bound, exact evaluation of the survivors, fold (prune_kernels.h)
     if ((rc = launch(c, s, KF_BOUND_TABLE, [&] {
            hipLaunchKernelGGL(k_bound_table_ref, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
@@ -777,14 +788,20 @@ int capture(hipStream_t s, hipGraphExec_t* exec, Q&& q) {
 // One encode step over the prepared batch.
 int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
   const bool graphs = c->tune.use_graph && !c->prof && fit;
+  Graphs& gr = c->graphs.v[c->any_inter ? 1 : 0];
   int rc;
-  if (graphs && !c->graphs.prefix) {
-    if ((rc = capture(s, &c->graphs.prefix, [&] { return queue_prefix(c, s, true); }))) return rc;
-    if ((rc = capture(s, &c->graphs.passes, [&] { return queue_passes(c, s); }))) return rc;
-    if ((rc = capture(s, &c->graphs.suffix, [&] { return queue_suffix(c, s); }))) return rc;
+  // Passes that change words write their count to h_conv; a step whose first
+  // search changed nothing has no such pass (its passes return at once).  No
+  // pass of an earlier step can still write it: they ended at a zero count,
+  // and the passes queued after that one return at once.
+  *reinterpret_cast<volatile uint32_t*>(c->h_conv) = 0u;
+  if (graphs && !gr.prefix) {
+    if ((rc = capture(s, &gr.prefix, [&] { return queue_prefix(c, s, true); }))) return rc;
+    if ((rc = capture(s, &gr.passes, [&] { return queue_passes(c, s); }))) return rc;
+    if ((rc = capture(s, &gr.suffix, [&] { return queue_suffix(c, s); }))) return rc;
   }
   if (graphs) {
-    HIP_TRY(hipGraphLaunch(c->graphs.prefix, s));
+    HIP_TRY(hipGraphLaunch(gr.prefix, s));
   } else if ((rc = queue_prefix(c, s, fit))) {
     return rc;
   }
@@ -796,7 +813,7 @@ int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
   int passes = 0;
   auto queue_batch = [&](int slot) -> int {
     if (graphs) {
-      HIP_TRY(hipGraphLaunch(c->graphs.passes, s));
+      HIP_TRY(hipGraphLaunch(gr.passes, s));
     } else {
       const int r = queue_passes(c, s);
       if (r) return r;
@@ -811,7 +828,11 @@ int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
     // the raster-triangular system settles within one pass per block
     if (passes > c->total + 4 * np) return fail(MPTC_ESTATE, "intra fixpoint did not converge");
     if ((rc = queue_batch(slot ^ 1))) return rc;
-    HIP_TRY(hipEventSynchronize(c->batch_ev[slot]));
+    // poll (a blocking wait adds its wake-up latency to every batch)
+    hipError_t q;
+    while ((q = hipEventQuery(c->batch_ev[slot])) == hipErrorNotReady) {
+    }
+    if (q != hipSuccess) return fail(MPTC_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
     // Every pass ends by writing its count of changed words to h_conv, so once
     // the batch is done h_conv holds the count of its last pass or of a later
     // one.  A pass that changes nothing proves the fixpoint (every later pass
@@ -820,7 +841,7 @@ int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
     slot ^= 1;
   }
   if (graphs) {
-    HIP_TRY(hipGraphLaunch(c->graphs.suffix, s));
+    HIP_TRY(hipGraphLaunch(gr.suffix, s));
   } else if ((rc = queue_suffix(c, s))) {
     return rc;
   }
