@@ -48,6 +48,18 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Two finite fp32 values -> packed bf16 (lo in bits 0-15), round to nearest
+// even as f2bf, in one v_cvt_pk_bf16_f32.  For the per-frame kernels'
+// activations and pixel inputs (never NaN or infinite).
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+  const bf16x2_t b = __builtin_convertvector((f32x2_t){lo, hi}, bf16x2_t);
+  uint32_t u;
+  __builtin_memcpy(&u, &b, 4);
+  return u;
+}
+
 // ---------------------------------------------------------------------------
 // GEMM.  Element (m, k) of A is A[m * sam + k * sak]; (k, n) of B is
 // B[k * sbk + n * sbn]; elements outside M x K / K x N read as 0.  A workgroup
@@ -302,7 +314,7 @@ __device__ __forceinline__ void net_input_row(const BlockPx& b, uint16_t* __rest
     for (int e = 0; e < 4; ++e) {
       const uint32_t w = b.w[2 * q + (e >> 1)];
       const uint32_t lo = (w >> (16 * (e & 1))) & 0xFFu, hi = (w >> (16 * (e & 1) + 8)) & 0xFFu;
-      h[e] = (uint32_t)f2bf((float)lo * (1.0f / 255.0f)) | ((uint32_t)f2bf((float)hi * (1.0f / 255.0f)) << 16);
+      h[e] = pack_bf16x2((float)lo * (1.0f / 255.0f), (float)hi * (1.0f / 255.0f));
     }
     o[q] = make_uint4(h[0], h[1], h[2], h[3]);
   }

@@ -69,13 +69,18 @@ __global__ void __launch_bounds__(256) k_pack_infer(const float* __restrict__ w,
   if (i < kL3Rows) bias[2 * kNetHid + i] = i < kNetOut ? w[kOffB3 + i] : 0.0f;
 }
 
+// LDS swizzle of the weight images: chunk c of row r sits at c ^ (r & kSwz).
+// An MFMA operand read has 16 consecutive rows read the same chunk index
+// together; with kSwz = 15 they land on 16 different 16-byte bank groups.
+constexpr int kSwz = 15;
+
 // Copies `rows` rows of `chunks` 16-byte chunks into LDS, chunk c of row r at
-// r * chunks + (c ^ (r & 7)) (swizzled) or r * chunks + c.
+// r * chunks + (c ^ (r & kSwz)) (swizzled; chunks >= 16) or r * chunks + c.
 __device__ __forceinline__ void stage_rows(uint4* __restrict__ dst, const uint4* __restrict__ src, int rows,
                                            int chunks, bool swz) {
   for (int e = threadIdx.x; e < rows * chunks; e += blockDim.x) {
     const int r = e / chunks, c = e % chunks;
-    dst[r * chunks + (swz ? (c ^ (r & 7)) : c)] = src[e];
+    dst[r * chunks + (swz ? (c ^ (r & kSwz)) : c)] = src[e];
   }
 }
 
@@ -92,7 +97,7 @@ __device__ __forceinline__ uint4 relu_pack(const f32x16_t& d, int half) {
   for (int q = 0; q < 4; ++q) {
     const int i0 = 8 * half + 2 * q, i1 = i0 + 1;
     const float v0 = fmaxf(d[i0], 0.0f), v1 = fmaxf(d[i1], 0.0f);
-    w[q] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+    w[q] = pack_bf16x2(v0, v1);
   }
   return make_uint4(w[0], w[1], w[2], w[3]);
 }
@@ -161,7 +166,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l12(const uint16_t* __
       const int r = nb * 32 + m;
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
-        const bf16x8_t a = as_bf16x8(sW2[r * 32 + ((2 * s + h) ^ (r & 7))]);
+        const bf16x8_t a = as_bf16x8(sW2[r * 32 + ((2 * s + h) ^ (r & kSwz))]);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, hb[s], acc, 0, 0, 0);
       }
       out[(2 * nb) * 64 + lane] = relu_pack(acc, 0);
@@ -311,7 +316,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
 #pragma unroll
       for (int nb = 0; nb < 9; ++nb) {
         const int r = nb * 32 + m;
-        const bf16x8_t a = as_bf16x8(sW3[r * 32 + ((2 * s + h) ^ (r & 7))]);
+        const bf16x8_t a = as_bf16x8(sW3[r * 32 + ((2 * s + h) ^ (r & kSwz))]);
         acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nb], 0, 0, 0);
       }
     }
