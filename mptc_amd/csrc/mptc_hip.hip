@@ -104,14 +104,14 @@ int init_tables(int device) {
 enum KernelId {
   KF_FIT, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
   KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
-  KF_NET_L3, KF_BOUND_TABLE, KF_PRUNED_INTER, KF_FOLD_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
+  KF_NET_L3, KF_BOUND_TABLE, KF_PRUNED_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
 };
 const char* kKernelNames[KF_N] = {"fit",          "inter",
                                   "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
                                   "intra_apply",  "intra_reeval", "intra_select", "commit",
                                   "unique",       "net_input",   "net_forward", "propose_inter",
                                   "top_intra",    "lintra_pass",  "net_l12",     "net_l3",
-                                  "bound_table",  "pruned_inter", "fold_inter",  "pruned_intra",
+                                  "bound_table",  "pruned_inter", "pruned_intra",
                                   "fold_intra"};
 constexpr int kMaxChunks = 8;  // window-row chunks per block in the search kernels
 
@@ -212,7 +212,6 @@ struct mptc_ctx {
   int16_t* d_pinter = nullptr;  // inter proposals, kMaxTopK per block
   int16_t* d_prop = nullptr;    // intra proposals, kMaxTopK per block
   // pruned exhaustive search (prune_kernels.h), allocated on first use
-  uint2* d_Ei = nullptr;        // inter error deltas, layout of d_E
   BoundTab *d_rbt = nullptr, *d_cbt = nullptr;  // bound tables of reference / current words
   bool record = false;          // exhaustive steps record acceptance masks
   bool last_recorded = false;   // the last step did
@@ -543,12 +542,10 @@ bool use_prune(const mptc_ctx* c) {
 }
 
 int prune_ensure(mptc_ctx* c) {
-  if (c->d_Ei) return MPTC_OK;
+  if (c->d_rbt) return MPTC_OK;
   const size_t nb = (size_t)c->max_total;
-  const size_t s = (size_t)std::min(c->alloc_S, kPruneMaxS);  // the pruned search runs for S <= kPruneMaxS
   int rc;
-  if ((rc = dalloc(&c->d_Ei, s * s * c->nbs)) || (rc = dalloc(&c->d_rbt, nb)) || (rc = dalloc(&c->d_cbt, nb)))
-    return rc;
+  if ((rc = dalloc(&c->d_rbt, nb)) || (rc = dalloc(&c->d_cbt, nb))) return rc;
   return MPTC_OK;
 }
 
@@ -636,14 +633,13 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
       return rc;
     if ((rc = launch(c, s, KF_PRUNED_INTER, [&] {
            hipLaunchKernelGGL(k_pruned_search<false>, dim3(gwk256), dim3(256), 0, s, c->d_fr, c->d_tsid, c->n_wtiles,
-                              tw, K, S, thr, c->d_own, c->d_need, c->d_rw, c->d_rtab, c->d_rbt, c->d_orig, c->d_Ei,
-                              c->nbs, c->d_stats);
+                              tw, K, S, thr, c->d_own, c->d_need, c->d_rw, c->d_rtab, c->d_rbt, c->d_orig, nullptr,
+                              c->nbs, c->d_summ, c->d_stats);
          })))
       return rc;
-    if ((rc = launch(c, s, KF_FOLD_INTER, [&] {
-           hipLaunchKernelGGL(k_fold_search<false>, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
-                              total, S, thr, c->d_own, c->d_need, c->d_rw, c->d_Ei, c->nbs, c->d_inter, c->d_cwn,
-                              c->d_chg, c->d_stats);
+    if ((rc = launch(c, s, KF_INTER_COMBINE, [&] {
+           hipLaunchKernelGGL(k_inter_combine, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
+                              K, S, thr, c->d_rw, c->d_summ, c->nbs, c->d_inter);
          })))
       return rc;
   } else if (c->mode == MPTC_SEARCH_EXHAUSTIVE) {
@@ -684,11 +680,11 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
     if ((rc = launch(c, s, KF_PRUNED_INTRA, [&] {
            hipLaunchKernelGGL(k_pruned_search<true>, dim3(gwk256), dim3(256), 0, s, c->d_fr, c->d_tsid, c->n_wtiles,
                               tw, K, S, thr, c->d_own, c->d_need, c->d_cw, c->d_ctab, c->d_cbt, c->d_orig, c->d_E,
-                              c->nbs, c->d_stats);
+                              c->nbs, nullptr, c->d_stats);
          })))
       return rc;
     if ((rc = launch(c, s, KF_FOLD_INTRA, [&] {
-           hipLaunchKernelGGL(k_fold_search<true>, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
+           hipLaunchKernelGGL(k_fold_intra, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
                               total, S, thr, c->d_own, c->d_need, c->d_cw, c->d_E, c->nbs, c->d_intra, c->d_cwn,
                               c->d_chg, c->d_stats);
          })))
@@ -952,7 +948,7 @@ void mptc_destroy(mptc_ctx* c) {
   c->nw.release();
   c->ew.release();
   void* lbufs[] = {c->d_h2p, c->d_pinter, c->d_prop, c->d_seen, c->d_errs, c->d_amask[0], c->d_amask[1],
-                   c->d_Ei,  c->d_rbt,    c->d_cbt};
+                   c->d_rbt,  c->d_cbt};
   for (void* p : lbufs)
     if (p) (void)hipFree(p);
   if (c->h_stats) (void)hipHostFree(c->h_stats);

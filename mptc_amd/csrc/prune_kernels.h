@@ -18,12 +18,14 @@
 // sums (v_dot4 over byte masks of the word) and one fp32 division per group,
 // with a 64 margin covering the fp32 rounding (< 48).
 //
-//   k_pruned_search wave tiles as the search kernels; per position the bound;
-//                   non-passing -> kPruned in E; the wave's survivors are
-//                   compacted in LDS and evaluated exactly, 64 per round
-//   k_fold_search   per block, the reference's selection over the passing
-//                   deltas of E: the inter decision, or the intra decision
-//                   with the changed-word list (as k_intra_combine)
+//   k_pruned_search wave tiles as the search kernels; per position the bound,
+//                   non-passing -> kPruned; the wave's survivors are compacted
+//                   in LDS and evaluated exactly, 64 per round.  Inter: the
+//                   chunk summaries of the selection rule (as k_inter, then
+//                   k_inter_combine); intra: E for the fixpoint passes
+//   k_fold_intra    per block, the reference's selection over the passing
+//                   deltas of E: the intra decision with the changed-word list
+//                   (as k_intra_combine)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -144,6 +146,32 @@ __device__ __forceinline__ bool window_block(int r, int t, int S, int bx, int by
 
 constexpr int kPruneBatch = 16;  // window positions per bound / evaluate round of a wave
 constexpr int kPackedBlk = 15;   // LDS words per block: pc[12], sum of squares, own word, own error
+constexpr int kFoldWords = 8;    // LDS words of an inter lane's fold: ChunkFold's 7 fields and its row
+
+// An inter lane's ChunkFold lives in LDS between the (rare) batches that
+// have passing deltas, so that it holds no registers through the rest.  Its
+// fields fit 16 bits: positions < 1024, rows < 64, and min_val is a passing
+// delta (<= thr) or the initial "none" (stored as kNoCand).
+__device__ __forceinline__ void cf_load(int16_t (*s)[64], int lane, ChunkFold& cf, int& row) {
+  cf.first_le0 = s[0][lane];
+  cf.le0_row = s[1][lane];
+  cf.after_sel = s[2][lane];
+  cf.last_neg = s[3][lane];
+  cf.min_val = s[4][lane] == kNoCand ? 0x7fffffff : (int)s[4][lane];
+  cf.min_pos = s[5][lane];
+  cf.row_neg = s[6][lane];
+  row = s[7][lane];
+}
+__device__ __forceinline__ void cf_store(int16_t (*s)[64], int lane, const ChunkFold& cf, int row) {
+  s[0][lane] = (int16_t)cf.first_le0;
+  s[1][lane] = (int16_t)cf.le0_row;
+  s[2][lane] = (int16_t)cf.after_sel;
+  s[3][lane] = (int16_t)cf.last_neg;
+  s[4][lane] = (int16_t)min(cf.min_val, (int)kNoCand);
+  s[5][lane] = (int16_t)cf.min_pos;
+  s[6][lane] = (int16_t)cf.row_neg;
+  s[7][lane] = (int16_t)row;
+}
 
 // Pruned search over the window of every searching block (inter: blocks of
 // inter frames against the previous frame's words; intra: blocks with need[]
@@ -151,8 +179,12 @@ constexpr int kPackedBlk = 15;   // LDS words per block: pc[12], sum of squares,
 // kernels.  Per batch of kPruneBatch positions each lane bounds its block's
 // candidates; the survivors of the whole wave are compacted in LDS and
 // evaluated exactly 64 at a time, a lane taking any block's survivor (the
-// pixel side of the wave's 64 blocks is kept in LDS).  E gets the exact delta
-// of survivors, kPruned for pruned and kNoCand for absent positions.
+// pixel side of the wave's 64 blocks is kept in LDS).  Each position's value
+// is the exact delta of a survivor, kPruned for a pruned and kNoCand for an
+// absent position.  Intra: the values go to E.  Inter: each lane folds its
+// chunk's passing values in scan order into the chunk summary summ[chunk][g]
+// of the dense inter kernel (for thr >= 0 the selection depends only on the
+// passing deltas, and a row without one leaves a summary as it is).
 template <bool kIntra>
 __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ tsid,
                                                        int n_wtiles, int tw, int n_chunks, int S, int thr,
@@ -161,12 +193,15 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
                                                        const uint32_t* __restrict__ words,
                                                        const float4* __restrict__ tabs,
                                                        const BoundTab* __restrict__ btab, const int* __restrict__ orig,
-                                                       uint2* __restrict__ E4, size_t nbs, DevStats* __restrict__ st) {
+                                                       uint2* __restrict__ E4, size_t nbs, uint4* __restrict__ summ,
+                                                       DevStats* __restrict__ st) {
   __shared__ uint8_t s_snap[512];
   __shared__ float2 s_ab[4];
   __shared__ uint32_t s_blk[4][kPackedBlk][64];      // [wave][field][lane]
   __shared__ int16_t s_e[4][kPruneBatch][64];        // the batch's E values, [wave][position][lane]
   __shared__ uint16_t s_list[4][kPruneBatch * 64 + 64];  // survivors: lane | position << 6; + per-lane dummies
+  __shared__ uint8_t s_pass[4][64];                      // inter: the lane's batch holds a passing delta
+  __shared__ int16_t s_cf[4][kFoldWords][64];            // inter: each lane's ChunkFold state and row
   __shared__ unsigned long long s_acc;
   load_lds_tables(s_snap, s_ab);
   int f = 0, bx = 0, by = 0, chunk = 0;
@@ -179,11 +214,11 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
   // tw is a power of two
   const int tw_log = __builtin_ctz((unsigned)tw);
   const int bx0 = bx - (lane & (tw - 1)), by0 = by - (lane >> tw_log);
-  BoundBlk b{};  // zero for lanes that do not search: their bounds are computed and ignored
   if (act) {
     BlockPx px;
     load_block(d.rgb, d.width, bx, by, px);
     const int e_own = kIntra ? orig[g] : own_error(px, own[g]);
+    BoundBlk b;
     make_bound_blk(px, e_own, thr, b);
 #pragma unroll
     for (int k = 0; k < 12; ++k) s_blk[wave][k][lane] = b.pc[k];
@@ -203,10 +238,35 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
   const bool wave_act = __ballot(act) != 0ull;
   // the frame's fields in registers (the loops' stores could otherwise alias them)
   const int fbw = d.bw, fbh = d.bh, fbase = d.base;
+  // inter: the selection rule over the chunk's positions in scan order
+  if (!kIntra) {
+    ChunkFold cf;
+    cf.begin_row();
+    cf_store(s_cf[wave], lane, cf, rb);
+  }
   for (int pb = p0; wave_act && pb < p1; pb += kPruneBatch) {
     const int nb = min(kPruneBatch, p1 - pb);  // a multiple of 4
     uint32_t cnt = 0;                          // wave-uniform
     int r = rb, t = tb;
+    if (!kIntra) s_pass[wave][lane] = 0;
+    // the block's bound constants, rebuilt from LDS for each batch so that they
+    // hold no registers through the evaluations (zero for lanes that do not
+    // search: their bounds are computed and ignored)
+    BoundBlk b{};
+    if (act) {
+#pragma unroll
+      for (int w = 0; w < 12; ++w) b.pc[w] = s_blk[wave][w][lane];
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        uint32_t tot = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tot = __builtin_amdgcn_udot4(b.pc[3 * j + ch], 0x01010101u, tot, false);
+        b.tot[ch] = (int)tot;
+      }
+      b.sq = s_blk[wave][12][lane];
+      b.q = (float)b.sq;
+      b.lim = bound_limit(thr, (int)s_blk[wave][14][lane]);
+    }
     for (int k = 0; k < nb; ++k) {
       int j = 0, i = 0;
       const bool there = act & window_block<kIntra>(r, t, S, bx, by, fbw, fbh, j, i);
@@ -253,25 +313,58 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
         const float4 t4 = tabs[q];
         const CandResult res = eval_candidate<false>(be, words[q], CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap, s_ab);
         s_e[wave][k][lb] = res.valid ? sat16(res.err) : kNoCand;
+        if (!kIntra && res.valid && res.err <= thr) s_pass[wave][lb] = 1;  // (equal writes may race)
         ++n_exact;
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (act)
+    if (act && kIntra) {  // intra: E, for the fixpoint passes
       for (int k = 0; k < nb; k += 4) {
         const uint32_t lo = (uint32_t)(uint16_t)s_e[wave][k][lane] | ((uint32_t)(uint16_t)s_e[wave][k + 1][lane] << 16);
         const uint32_t hi =
             (uint32_t)(uint16_t)s_e[wave][k + 2][lane] | ((uint32_t)(uint16_t)s_e[wave][k + 3][lane] << 16);
         E4[(size_t)((pb + k) >> 2) * nbs + g] = make_uint2(lo, hi);
       }
+    } else if (!kIntra && __ballot(act && s_pass[wave][lane] != 0) != 0ull) {
+      // inter: fold the batch's passing deltas in scan order.  The selection
+      // depends only on them (thr >= 0), and a row without one leaves the
+      // summary as it is, so the other positions are skipped.
+      if (act && s_pass[wave][lane] != 0) {
+        ChunkFold cf;
+        int cf_row;
+        cf_load(s_cf[wave], lane, cf, cf_row);
+        int rr = rb, tt = tb;
+        for (int k = 0; k < nb; ++k) {
+          const int v = s_e[wave][k][lane];
+          if (v != kNoCand && v <= thr) {
+            if (rr != cf_row) {
+              cf.end_row(cf_row);
+              cf.begin_row();
+              cf_row = rr;
+            }
+            cf.step(v, pb + k, rr);
+          }
+          tt = tt + 1 == W2 ? 0 : tt + 1;
+          rr += tt == 0 ? 1 : 0;
+        }
+        cf_store(s_cf[wave], lane, cf, cf_row);
+      }
+    }
     // the next batch overwrites s_e and s_list
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     rb = r;
     tb = t;
+  }
+  if (act && !kIntra) {
+    ChunkFold cf;
+    int cf_row;
+    cf_load(s_cf[wave], lane, cf, cf_row);
+    cf.end_row(cf_row);
+    summ[(size_t)chunk * nbs + g] = cf.pack();
   }
   wg_count(&s_acc, n_bound, kIntra ? st->intra_cands : st->inter_cands);
   wg_count(&s_acc, n_exact, kIntra ? st->exact_intra : st->exact_inter);
@@ -316,45 +409,31 @@ __device__ __forceinline__ Fold fold_passing(const uint2* __restrict__ E4, size_
   return fd;
 }
 
-// Decisions from E.  Inter: the accept test (:890) -> dec (as k_inter_combine).
-// Intra: as k_intra_combine -> dec, the new current word and the changed list.
-template <bool kIntra>
-__global__ void __launch_bounds__(256) k_fold_search(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
-                                                     int total, int S, int thr, const uint64_t* __restrict__ own,
-                                                     const uint8_t* __restrict__ need,
-                                                     const uint32_t* __restrict__ words,
-                                                     const uint2* __restrict__ E4, size_t nbs,
-                                                     uint64_t* __restrict__ dec, uint32_t* __restrict__ cw_new,
-                                                     int* __restrict__ chg_list, DevStats* __restrict__ st) {
+// Intra decisions from E (as k_intra_combine): dec, the new current word and
+// the first changed-word list.
+__global__ void __launch_bounds__(256) k_fold_intra(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
+                                                    int total, int S, int thr, const uint64_t* __restrict__ own,
+                                                    const uint8_t* __restrict__ need,
+                                                    const uint32_t* __restrict__ words,
+                                                    const uint2* __restrict__ E4, size_t nbs,
+                                                    uint64_t* __restrict__ dec, uint32_t* __restrict__ cw_new,
+                                                    int* __restrict__ chg_list, DevStats* __restrict__ st) {
   __shared__ unsigned int s_base;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   bool changed = false;
-  if (g < total) {
+  if (g < total && need[g]) {
     const FrameDev& d = fr[sid[g]];
     const int k = g - d.base, bx = k % d.bw, by = k / d.bw;
-    if (kIntra) {
-      if (need[g]) {
-        const Fold fd = fold_passing(E4, nbs, g, S, thr);
-        const uint64_t dc = intra_decision(words, d.base, d.bw, bx, by, S, thr, fd);
-        dec[g] = dc;
-        const uint32_t w = dec_acc(dc) ? dec_word(dc) : (uint32_t)(own[g] >> 32);
-        if (w != words[g]) {
-          cw_new[g] = w;
-          changed = true;
-        }
-      }
-    } else if (!d.is_intra) {
-      const Fold fd = fold_passing(E4, nbs, g, S, thr);
-      uint64_t dc = 0;
-      if (fd.sel >= 0 && fd.min_err <= thr) {
-        const int W2 = 2 * S, r = fd.sel / W2, t = fd.sel % W2;
-        dc = pack_dec(words[d.base + (by - S + r) * d.bw + (bx - S + t)], (uint32_t)t | 0x80u,
-                      (uint32_t)r | 0x80u, true);
-      }
-      dec[g] = dc;
+    const Fold fd = fold_passing(E4, nbs, g, S, thr);
+    const uint64_t dc = intra_decision(words, d.base, d.bw, bx, by, S, thr, fd);
+    dec[g] = dc;
+    const uint32_t w = dec_acc(dc) ? dec_word(dc) : (uint32_t)(own[g] >> 32);
+    if (w != words[g]) {
+      cw_new[g] = w;
+      changed = true;
     }
   }
-  if (kIntra) wg_append(changed, g, chg_list, &st->n_changed[0], &s_base);
+  wg_append(changed, g, chg_list, &st->n_changed[0], &s_base);
 }
 
 // Bound tables of the previous frames' words (inter candidates).
