@@ -435,7 +435,9 @@ __device__ __forceinline__ void init_ab_table(float2* s_ab) {
   }
 }
 
-template <bool kWantEps>
+// kFromBytes: the pixel channels are converted from be.pc (the same exact
+// float values) instead of read from be.f, which then needs no registers.
+template <bool kWantEps, bool kFromBytes = false>
 __device__ __forceinline__ CandResult eval_candidate(const BlockEval& be, uint32_t c, const CandTab& t,
                                                      const uint8_t* __restrict__ snap,
                                                      const float2* __restrict__ s_ab) {
@@ -460,7 +462,9 @@ __device__ __forceinline__ CandResult eval_candidate(const BlockEval& be, uint32
     for (int n = 0; n < 4; ++n) {
       const int i = 4 * j + n;
       const float2 w = *reinterpret_cast<const float2*>(ab_base + ((s8 >> (8 * n)) & 0xFFu));
-      const float p0 = be.f[3 * i + 0], p1 = be.f[3 * i + 1], p2 = be.f[3 * i + 2];
+      const float p0 = kFromBytes ? (float)((be.pc[3 * j + 0] >> (8 * n)) & 0xFFu) : be.f[3 * i + 0];
+      const float p1 = kFromBytes ? (float)((be.pc[3 * j + 1] >> (8 * n)) & 0xFFu) : be.f[3 * i + 1];
+      const float p2 = kFromBytes ? (float)((be.pc[3 * j + 2] >> (8 * n)) & 0xFFu) : be.f[3 * i + 2];
       if (i == 0) {
         ax0 = p0 * w.x; ax1 = p1 * w.x; ax2 = p2 * w.x;
         bx0 = p0 * w.y; bx1 = p1 * w.y; bx2 = p2 * w.y;

@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
             wj = w[i];
             fresh = er[i] == kNotEvaluated;
           }
-        const CandResult res = eval_candidate<false>(be, wj, cand_table(wj), s_snap, s_ab);
+        const CandResult res = eval_candidate<false, true>(be, wj, cand_table(wj), s_snap, s_ab);
         const int e = res.valid ? res.err : 0x7fffffff;
         if (fresh)
           ++n_cand;

@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(256) k_eval_inter(const FrameDev* __restrict__
         const int q = d.base + (by + L / kNetWin - 8) * bw + (bx + L % kNetWin - 8);
         const float4 t4 = rtab[q];
         const uint32_t c = rw[q];
-        const CandResult res = eval_candidate<false>(be, c, CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap, s_ab);
+        const CandResult res = eval_candidate<false, true>(be, c, CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap, s_ab);
         ++n_cand;
         if (res.valid && res.err < best) {
           best = res.err;

@@ -300,18 +300,15 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
         int j = 0, i = 0;
         window_block<kIntra>(re, te, S, bx0 + (lb & (tw - 1)), by0 + (lb >> tw_log), fbw, fbh, j, i);
         const int q = fbase + j * fbw + i;
-        BlockEval be;
+        BlockEval be;  // the pixel side from LDS; eval_candidate converts the bytes itself
 #pragma unroll
         for (int w = 0; w < 12; ++w) be.pc[w] = s_blk[wave][w][lb];
-#pragma unroll
-        for (int p = 0; p < 16; ++p)
-#pragma unroll
-          for (int ch = 0; ch < 3; ++ch) be.f[3 * p + ch] = (float)((be.pc[3 * (p >> 2) + ch] >> (8 * (p & 3))) & 0xFFu);
         be.sq = s_blk[wave][12][lb];
         be.own = s_blk[wave][13][lb];
         be.orig = (int)s_blk[wave][14][lb];
         const float4 t4 = tabs[q];
-        const CandResult res = eval_candidate<false>(be, words[q], CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap, s_ab);
+        const CandResult res =
+            eval_candidate<false, true>(be, words[q], CandTab{t4.x, t4.y, t4.z, t4.w}, s_snap, s_ab);
         s_e[wave][k][lb] = res.valid ? sat16(res.err) : kNoCand;
         if (!kIntra && res.valid && res.err <= thr) s_pass[wave][lb] = 1;  // (equal writes may race)
         ++n_exact;
