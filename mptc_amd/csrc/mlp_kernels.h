@@ -477,6 +477,24 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
         const uint32_t v = __shfl(cur, src[i] >= 0 ? src[i] : lane);
         if (src[i] >= 0) w[i] = v;
       }
+      // A word some slot has evaluated for this block (sn, er: the error of
+      // that word, whether or not the slot's own proposal still holds it)
+      // needs no new evaluation: neighbouring blocks often share index words,
+      // and a word moving along a chain of reuse arrives in several slots.
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        if (Ls[i] >= 0 && (er[i] == kNotEvaluated || w[i] != sn[i])) {
+          int hit = kNotEvaluated;
+#pragma unroll
+          for (int k2 = 0; k2 < K; ++k2)
+            if (k2 != i && er[k2] != kNotEvaluated && sn[k2] == w[i]) hit = er[k2];
+          if (hit != kNotEvaluated) {
+            sn[i] = w[i];
+            er[i] = hit;
+            clean_once = false;  // decide again when clean
+          }
+        }
+      }
       int j = -1;  // first proposal whose evaluation is stale
 #pragma unroll
       for (int i = K - 1; i >= 0; --i)
