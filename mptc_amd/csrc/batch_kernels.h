@@ -332,7 +332,9 @@ __global__ void __launch_bounds__(256) k_inter_combine(const FrameDev* __restric
 
 // Blocks that need the intra search, the current words the search starts
 // from (inter-reused words are final: they read only the previous frame;
-// every other block starts from its own word), and each block's own error.
+// every other block starts from its own word), and for the exact intra
+// search (orig not null) each block's own error, the current words' tables
+// and cleared pass flags.
 __global__ void __launch_bounds__(256) k_intra_prep(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid, int total,
                              const uint64_t* __restrict__ own, const uint64_t* __restrict__ inter_dec,
                              uint8_t* __restrict__ need, uint32_t* __restrict__ cw, float4* __restrict__ ctab,
@@ -352,8 +354,9 @@ __global__ void __launch_bounds__(256) k_intra_prep(const FrameDev* __restrict__
     }
   }
   need[g] = inter_ok ? 0 : 1;
-  dirty[g] = 0;
   cw[g] = w;
+  if (!orig) return;  // the learned intra search builds its own tables and errors
+  dirty[g] = 0;
   const CandTab t = cand_table(w);
   ctab[g] = make_float4(t.asq, t.bsq, t.ab, t.f);
   BlockPx px;

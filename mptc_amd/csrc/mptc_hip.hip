@@ -663,8 +663,10 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
       return rc;
   }
   if ((rc = launch(c, s, KF_PREP, [&] {
+         const bool lrn = c->mode == MPTC_SEARCH_LEARNED;  // its intra passes use no tables, errors or flags
          hipLaunchKernelGGL(k_intra_prep, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
-                            c->d_own, c->d_inter, c->d_need, c->d_cw, c->d_ctab, c->d_dirty, c->d_orig);
+                            c->d_own, c->d_inter, c->d_need, c->d_cw, lrn ? nullptr : c->d_ctab,
+                            lrn ? nullptr : c->d_dirty, lrn ? nullptr : c->d_orig);
        })))
     return rc;
   if (c->mode == MPTC_SEARCH_LEARNED) {
