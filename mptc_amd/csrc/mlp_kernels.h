@@ -470,6 +470,38 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
       published = true;
       clean_once = true;
     };
+    if (first) {
+      // The first pass evaluates every proposal against the words the pass
+      // starts from in one go (a word already evaluated in an earlier slot is
+      // taken from it); the rounds below then only follow changes.  The
+      // fixpoint is unique, so the order of evaluations does not change it.
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const uint32_t v = __shfl(cur, src[i] >= 0 ? src[i] : lane);
+        if (src[i] >= 0) w[i] = v;
+      }
+      if (act) {
+        BlockPx px;
+        load_block(d.rgb, d.width, bx, by, px);
+        make_block_eval(px, own[g], be);
+        have_be = true;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          if (Ls[i] < 0) continue;
+          int hit = kNotEvaluated;
+#pragma unroll
+          for (int k2 = 0; k2 < i; ++k2)
+            if (Ls[k2] >= 0 && sn[k2] == w[i]) hit = er[k2];
+          if (hit == kNotEvaluated) {
+            const CandResult res = eval_candidate<false, true>(be, w[i], cand_table(w[i]), s_snap, s_ab);
+            hit = res.valid ? res.err : 0x7fffffff;
+            ++n_cand;
+          }
+          sn[i] = w[i];
+          er[i] = hit;
+        }
+      }
+    }
     for (int round = 0; round < kMaxRounds; ++round) {
       // current words of the proposals inside the tile
 #pragma unroll
