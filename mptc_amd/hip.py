@@ -52,9 +52,9 @@ def synchronize():
 
 def device_name(d: int = 0) -> str:
     buf = C.create_string_buffer(256)
-    if rt().hipDeviceGetName(buf, 256, d) == 0:
+    if rt().hipDeviceGetName(buf, 256, d) == 0 and buf.value:
         return buf.value.decode()
-    return "unknown"
+    return f"HIP device {d}"  # the runtime may report no marketing name
 
 
 class Buffer:
