@@ -404,6 +404,79 @@ __device__ __forceinline__ int e_at(uint2 g, int n) {
   return (int)(int16_t)((n & 1) ? (w >> 16) : (w & 0xFFFFu));
 }
 
+// The selection rule (codec/dxt_image.cpp:697-708) over the passing deltas of
+// block g's E (for thr >= 0 that is the full rule's result: before the first
+// passing delta the others only raise the running minimum, which that delta
+// then undercuts; after it they can never update it).  The S^2 groups of four
+// are loaded 32 at a time.
+__device__ __forceinline__ Fold fold_passing(const uint2* __restrict__ E4, size_t nbs, int g, int S, int thr) {
+  const int W2 = 2 * S, groups = S * S;
+  Fold fd;
+  int row = -1;
+  for (int g0 = 0; g0 < groups; g0 += 32) {
+    uint2 buf[32];
+#pragma unroll
+    for (int b = 0; b < 32; ++b)  // past the last group: absent candidates
+      buf[b] = g0 + b < groups ? E4[(size_t)(g0 + b) * nbs + g] : make_uint2(0x7FFF7FFFu, 0x7FFF7FFFu);
+#pragma unroll
+    for (int b = 0; b < 32; ++b) {
+      const uint2 gg = buf[b];
+      bool any = false;
+#pragma unroll
+      for (int nn = 0; nn < 4; ++nn) {
+        const int e = e_at(gg, nn);
+        any = any || (e != kNoCand && e <= thr);
+      }
+      if (!any) continue;
+#pragma unroll
+      for (int nn = 0; nn < 4; ++nn) {
+        const int e = e_at(gg, nn);
+        if (e == kNoCand || e > thr) continue;
+        const int pos = 4 * (g0 + b) + nn;
+        const int r = pos / W2;
+        if (r != row) {
+          row = r;
+          fd.row_done = false;
+        }
+        fd.step(e, pos);
+      }
+    }
+  }
+  return fd;
+}
+
+// Passing sets: for windows of at most 256 positions (S <= 8), bit p % 32 of
+// word p / 32 of pm[8 g .. 8 g + 7] is set exactly when block g's delta at
+// position p passes (valid, <= thr).  Folding a block then reads its 32 bytes
+// of set and the few passing deltas instead of all of its E.
+constexpr int kPassWords = 8;
+
+__device__ __forceinline__ Fold fold_passing_set(const uint32_t* __restrict__ pm, const uint2* __restrict__ E4,
+                                                 size_t nbs, int g, int S) {
+  const int W2 = 2 * S;
+  const uint4* p4 = reinterpret_cast<const uint4*>(pm + (size_t)g * kPassWords);
+  const uint4 a = p4[0], b = p4[1];
+  const uint32_t m[kPassWords] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  Fold fd;
+  int row = -1;
+#pragma unroll
+  for (int w = 0; w < kPassWords; ++w) {
+    uint32_t mw = m[w];
+    while (mw != 0u) {  // set bits in increasing position: scan order
+      const int pos = 32 * w + __builtin_ctz(mw);
+      mw &= mw - 1u;
+      const int e = e_at(E4[(size_t)(pos >> 2) * nbs + g], pos & 3);
+      const int r = pos / W2;
+      if (r != row) {
+        row = r;
+        fd.row_done = false;
+      }
+      fd.step(e, pos);
+    }
+  }
+  return fd;
+}
+
 __global__ void __launch_bounds__(256) k_intra_dense(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ tsid,
                                                      int n_wtiles, int tw, int n_chunks, int S,
                                                      const uint64_t* __restrict__ own, const uint8_t* __restrict__ need,
@@ -511,15 +584,16 @@ __global__ void __launch_bounds__(256) k_intra_apply(const uint32_t* __restrict_
 // a word changed in the previous step is re-evaluated with the new word.
 // Work item = (changed position, slot); when 64 divides the 4S^2 slots every
 // wave serves one changed position.  Blocks whose fold may change are flagged
-// for part 2 (plain stores: racing writers all store 1).  Reads the changed
-// list of parity `par` and clears the one of the next pass.
+// for part 2 (plain stores: racing writers all store 1), and the passing sets
+// pm (when kept) follow E.  Reads the changed list of parity `par` and clears
+// the one of the next pass.
 __global__ void __launch_bounds__(256) k_intra_reeval(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
                                                       int S, int thr, const uint64_t* __restrict__ own,
                                                       const uint8_t* __restrict__ need, const uint32_t* __restrict__ cw,
                                                       const float4* __restrict__ ctab, const int* __restrict__ chg_list,
                                                       uint2* __restrict__ E4, size_t nbs, uint32_t* __restrict__ dirty,
                                                       const int* __restrict__ orig, const uint64_t* __restrict__ dec,
-                                                      DevStats* __restrict__ st, int par) {
+                                                      DevStats* __restrict__ st, int par, uint32_t* __restrict__ pm) {
   const int W2 = 2 * S, slots = W2 * W2;
   const unsigned n_chg = st->n_changed[par];
   if (blockIdx.x == 0 && threadIdx.x == 0) st->n_changed[par ^ 1] = 0;
@@ -570,6 +644,9 @@ __global__ void __launch_bounds__(256) k_intra_reeval(const FrameDev* __restrict
     if (old != ev) *slot = ev;
     const bool passes_old = old != kNoCand && old <= thr, passes_new = ev != kNoCand && ev <= thr;
     if ((old != ev && (passes_old || passes_new)) || is_sel) dirty[k] = 1u;
+    // one work item per (block, position) in a pass: bits of a word flip
+    // concurrently only for different positions
+    if (pm && passes_old != passes_new) atomicXor(&pm[(size_t)k * kPassWords + (s >> 5)], 1u << (s & 31));
   }
   wg_count(&s_acc, n_eval, st->reeval_cands);
 }
@@ -581,13 +658,15 @@ __global__ void __launch_bounds__(256) k_intra_reeval(const FrameDev* __restrict
 // pass re-evaluates every pair that sees the word, so the passes still end at
 // the same fixpoint.  Writes the changed list of parity `par` (cleared by the
 // preceding k_intra_reeval); its length goes to *conv when the pass ends.
+// pm: the passing sets (null: fold from E alone).
 __global__ void __launch_bounds__(256) k_intra_select(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
                                                       int total, int S, int thr, const uint64_t* __restrict__ own,
                                                       uint32_t* __restrict__ cw, float4* __restrict__ ctab,
                                                       const uint2* __restrict__ E4, size_t nbs,
                                                       uint32_t* __restrict__ dirty, uint64_t* __restrict__ dec,
                                                       int* __restrict__ chg_list, DevStats* __restrict__ st, int par,
-                                                      unsigned int* __restrict__ conv) {
+                                                      unsigned int* __restrict__ conv,
+                                                      const uint32_t* __restrict__ pm) {
   // The preceding k_intra_reeval flags blocks only from the changed list of
   // parity par ^ 1; when that list was empty there is nothing to re-fold.
   // No workgroup of this pass writes that counter, so either every workgroup
@@ -599,43 +678,9 @@ __global__ void __launch_bounds__(256) k_intra_select(const FrameDev* __restrict
   bool changed = false;
   if (k < total && dirty[k] != 0u) {
     dirty[k] = 0;
-    const int W2 = 2 * S, groups = S * S;
     // Only deltas that pass the threshold can change the decision (see
-    // k_intra_reeval), so the fold skips groups of four that hold none.
-    Fold fd;
-    int row = -1;
-    // the E loads of a block are independent: issue them in batches of 32
-    // (one memory round trip each) before folding
-    for (int g0 = 0; g0 < groups; g0 += 32) {
-      uint2 buf[32];
-#pragma unroll
-      for (int b = 0; b < 32; ++b)  // past the last group: absent candidates
-        buf[b] = g0 + b < groups ? E4[(size_t)(g0 + b) * nbs + k] : make_uint2(0x7FFF7FFFu, 0x7FFF7FFFu);
-#pragma unroll
-      for (int b = 0; b < 32; ++b) {
-      const int gi = g0 + b;
-      const uint2 gg = buf[b];
-      bool any = false;
-#pragma unroll
-      for (int nn = 0; nn < 4; ++nn) {
-        const int e = e_at(gg, nn);
-        any = any || (e != kNoCand && e <= thr);
-      }
-      if (!any) continue;
-#pragma unroll
-      for (int nn = 0; nn < 4; ++nn) {
-        const int e = e_at(gg, nn);
-        if (e == kNoCand || e > thr) continue;
-        const int pos = 4 * gi + nn;
-        const int r = pos / W2;
-        if (r != row) {
-          row = r;
-          fd.row_done = false;
-        }
-        fd.step(e, pos);
-      }
-      }
-    }
+    // k_intra_reeval): fold those, from the passing set where one is kept.
+    const Fold fd = pm ? fold_passing_set(pm, E4, nbs, k, S) : fold_passing(E4, nbs, k, S, thr);
     const FrameDev& d = fr[sid[k]];
     const int kl = k - d.base;
     const uint64_t dc = intra_decision(cw, d.base, d.bw, kl % d.bw, kl / d.bw, S, thr, fd);

@@ -67,6 +67,10 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
 // quadrant (w / 2, w % 2) as 2 x 2 MFMA 16x16x32 tiles.  MFMA operand layout
 // (lane l of the wave): A row l % 16, k (l / 16) * 8 .. +7; B column l % 16,
 // k (l / 16) * 8 .. +7; D rows (l / 16) * 4 .. +3, column l % 16.
+// Split K (gridDim.z > 1, for the weight gradients, whose K is the batch):
+// workgroup z takes a 32-aligned range of k and stores its partial product,
+// without epilogue, to Cf + z * split_stride; k_split_sum adds the partials
+// in a fixed order.
 // ---------------------------------------------------------------------------
 struct GemmEpi {
   const float* bias;     // [N] or null
@@ -77,6 +81,7 @@ struct GemmEpi {
   uint16_t* Cb;          // bf16 output or null
   long ldc, ldcb;        // row strides of Cf, Cb
   int trans_f;           // Cf stored transposed: Cf[n * ldc + m]
+  long split_stride;     // split K: floats between the partial products
 };
 
 __global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, long sam,
@@ -91,17 +96,19 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const ui
   f32x4_t acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 32) {
+  const int kspan = ((K + (int)gridDim.z - 1) / (int)gridDim.z + 31) & ~31;
+  const int kb = (int)blockIdx.z * kspan, ke = min(K, kb + kspan);
+  for (int k0 = kb; k0 < ke; k0 += 32) {
     // consecutive threads walk the contiguous dimension of each operand
     for (int e = tid; e < 64 * 32; e += 256) {
       const int mm = sak == 1 ? e / 32 : e % 64, kk = sak == 1 ? e % 32 : e / 64;
       const int m = m0 + mm, k = k0 + kk;
-      sA[mm][kk] = (m < M && k < K) ? A[(long)m * sam + (long)k * sak] : (uint16_t)0;
+      sA[mm][kk] = (m < M && k < ke) ? A[(long)m * sam + (long)k * sak] : (uint16_t)0;
     }
     for (int e = tid; e < 64 * 32; e += 256) {
       const int nn = sbn == 1 ? e % 64 : e / 32, kk = sbn == 1 ? e / 64 : e % 32;
       const int k = k0 + kk, n = n0 + nn;
-      sB[nn][kk] = (k < K && n < N) ? B[(long)k * sbk + (long)n * sbn] : (uint16_t)0;
+      sB[nn][kk] = (k < ke && n < N) ? B[(long)k * sbk + (long)n * sbn] : (uint16_t)0;
     }
     __syncthreads();
     const int r = lane & 15, kq = (lane >> 4) * 8;
@@ -115,6 +122,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const ui
     __syncthreads();
   }
   const int col_in = lane & 15, row_q = (lane >> 4) * 4;
+  float* const Cf = ep.Cf ? ep.Cf + (long)blockIdx.z * ep.split_stride : nullptr;
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j)
       for (int t = 0; t < 4; ++t) {
@@ -124,9 +132,20 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const ui
         if (ep.bias) v += ep.bias[n];
         if (ep.relu) v = fmaxf(v, 0.0f);
         if (ep.mask && !(bf2f(ep.mask[(long)m * ep.ldm + n]) > 0.0f)) v = 0.0f;
-        if (ep.Cf) ep.Cf[ep.trans_f ? (long)n * ep.ldc + m : (long)m * ep.ldc + n] = v;
+        if (Cf) Cf[ep.trans_f ? (long)n * ep.ldc + m : (long)m * ep.ldc + n] = v;
         if (ep.Cb) ep.Cb[(long)m * ep.ldcb + n] = f2bf(v);
       }
+}
+
+// C[m * ldc + n] = sum over z = 0 .. splits-1 of P[z * stride + m * N + n].
+__global__ void __launch_bounds__(256) k_split_sum(const float* __restrict__ P, int splits, long stride, int M, int N,
+                                                   float* __restrict__ C, long ldc) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)M * N) return;
+  const int m = (int)(i / N), n = (int)(i % N);
+  float s = 0.0f;
+  for (int z = 0; z < splits; ++z) s += P[(long)z * stride + i];
+  C[(long)m * ldc + n] = s;
 }
 
 // Network input: x = pixel / 255 (pixel-major RGB), bf16.  Row r of X takes
@@ -146,67 +165,75 @@ __global__ void __launch_bounds__(256) k_net_input(const uint8_t* __restrict__ p
 // the usual cross-entropy), plus seed_w * the mean squared error of the 6
 // seeds against the endpoints / 255.  d/dy_c of the set term is
 // p_c - [c in set] exp(y_c) / sum_{set} exp(y).  Writes dY = d(mean loss)/dY in
-// fp32 and bf16 and adds the batch's mean loss to *loss.  One wave per row;
-// target and eps rows are taken through idx like the inputs.  A row with an
-// empty set adds only its seed term.
+// fp32 and bf16 and adds the batch's mean loss to *loss.  A wave per row (rows
+// strided over the grid); target and eps rows are taken through idx like the
+// inputs.  A row with an empty set adds only its seed term.
 __global__ void __launch_bounds__(256) k_net_loss(const float* __restrict__ Y, int n, const uint32_t* __restrict__ tmask,
                                                   const uint8_t* __restrict__ eps, const int32_t* __restrict__ idx,
                                                   float seed_w, float* __restrict__ dYf, uint16_t* __restrict__ dYb,
                                                   float* __restrict__ loss) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (row >= n) return;
-  const long src = idx ? (long)idx[row] : row;
-  const float* y = Y + (size_t)row * kNetOut;
-  uint32_t tm[8];
+  __shared__ float s_l[4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float inv_n = 1.0f / (float)n;
+  float l = 0.0f;  // this lane's share of the wave's rows' losses
+  for (int row = blockIdx.x * 4 + wave; row < n; row += gridDim.x * 4) {
+    const long src = idx ? (long)idx[row] : row;
+    const float* y = Y + (size_t)row * kNetOut;
+    uint32_t tm[8];
 #pragma unroll
-  for (int w = 0; w < 8; ++w) tm[w] = tmask[src * 8 + w];
-  float mx = -3.0e38f;
-  for (int c = lane; c < kNetLogits; c += 64) mx = fmaxf(mx, y[c]);
-  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-  float se = 0.0f, ss = 0.0f;  // sum over all logits, over the set
-  for (int c = lane; c < kNetLogits; c += 64) {
-    const float e = expf(y[c] - mx);
-    se += e;
-    if ((tm[c >> 5] >> (c & 31)) & 1u) ss += e;
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    se += __shfl_xor(se, o);
-    ss += __shfl_xor(ss, o);
-  }
-  const float inv_n = 1.0f / (float)n, lse = logf(se);
-  const bool has_set = ss > 0.0f;
-  float l = has_set ? (lane == 0 ? lse - logf(ss) : 0.0f) : 0.0f;
-#pragma unroll
-  for (int j = 0; j < 5; ++j) {
-    const int c = lane + 64 * j;
-    if (c >= kNetOut) break;
-    float g;
-    if (c < kNetLogits) {
+    for (int w = 0; w < 8; ++w) tm[w] = tmask[src * 8 + w];
+    float mx = -3.0e38f;
+    for (int c = lane; c < kNetLogits; c += 64) mx = fmaxf(mx, y[c]);
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    float se = 0.0f, ss = 0.0f;  // sum over all logits, over the set
+    for (int c = lane; c < kNetLogits; c += 64) {
       const float e = expf(y[c] - mx);
-      const float q = (has_set && ((tm[c >> 5] >> (c & 31)) & 1u)) ? e / ss : 0.0f;
-      g = has_set ? (e / se - q) * inv_n : 0.0f;
-    } else {
-      const float t = (float)eps[src * kNetSeeds + (c - kNetLogits)] * (1.0f / 255.0f);
-      const float d = y[c] - t;
-      l += seed_w * d * d * (1.0f / kNetSeeds);
-      g = 2.0f * seed_w * d * (1.0f / kNetSeeds) * inv_n;
+      se += e;
+      if ((tm[c >> 5] >> (c & 31)) & 1u) ss += e;
     }
-    dYf[(size_t)row * kNetOut + c] = g;
-    dYb[(size_t)row * kNetOut + c] = f2bf(g);
+    for (int o = 32; o > 0; o >>= 1) {
+      se += __shfl_xor(se, o);
+      ss += __shfl_xor(ss, o);
+    }
+    const float lse = logf(se);
+    const bool has_set = ss > 0.0f;
+    if (has_set && lane == 0) l += lse - logf(ss);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const int c = lane + 64 * j;
+      if (c >= kNetOut) break;
+      float g;
+      if (c < kNetLogits) {
+        const float e = expf(y[c] - mx);
+        const float q = (has_set && ((tm[c >> 5] >> (c & 31)) & 1u)) ? e / ss : 0.0f;
+        g = has_set ? (e / se - q) * inv_n : 0.0f;
+      } else {
+        const float t = (float)eps[src * kNetSeeds + (c - kNetLogits)] * (1.0f / 255.0f);
+        const float d = y[c] - t;
+        l += seed_w * d * d * (1.0f / kNetSeeds);
+        g = 2.0f * seed_w * d * (1.0f / kNetSeeds) * inv_n;
+      }
+      dYf[(size_t)row * kNetOut + c] = g;
+      dYb[(size_t)row * kNetOut + c] = f2bf(g);
+    }
   }
+  // one atomic per workgroup: same-address atomics serialise
   for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
-  if (lane == 0) atomicAdd(loss, l * inv_n);
+  if (lane == 0) s_l[wave] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(loss, (s_l[0] + s_l[1] + s_l[2] + s_l[3]) * inv_n);
 }
 
 // Column sums in two deterministic stages: part[p][c] = sum of rows
-// p*256 .. p*256+255, then out[c] = sum over p in order.
+// p*kColsumRows .. (p+1)*kColsumRows - 1, then out[c] = sum over p in order.
+constexpr int kColsumRows = 32;  // a 4096-row batch: 128 partials per column
 __global__ void __launch_bounds__(256) k_colsum_part(const float* __restrict__ X, int rows, int cols, long ld,
                                                      float* __restrict__ part) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, p = blockIdx.y;
   if (c >= cols) return;
   float s = 0.0f;
-  const int r1 = min(rows, (p + 1) * 256);
-  for (int r = p * 256; r < r1; ++r) s += X[(long)r * ld + c];
+  const int r1 = min(rows, (p + 1) * kColsumRows);
+  for (int r = p * kColsumRows; r < r1; ++r) s += X[(long)r * ld + c];
   part[(long)p * cols + c] = s;
 }
 

@@ -156,8 +156,9 @@ struct NetWork {
   bool train = false;
   uint16_t *X = nullptr, *H1 = nullptr, *H2 = nullptr, *dYb = nullptr, *dH2b = nullptr, *dH1b = nullptr;
   float *Y = nullptr, *dYf = nullptr, *dH2f = nullptr, *dH1f = nullptr, *part = nullptr;
+  float* split = nullptr;  // split-K partial products of the weight gradients (kSplitFloats)
   void release() {
-    void* b[] = {X, H1, H2, dYb, dH2b, dH1b, Y, dYf, dH2f, dH1f, part};
+    void* b[] = {X, H1, H2, dYb, dH2b, dH1b, Y, dYf, dH2f, dH1f, part, split};
     for (void* p : b)
       if (p) (void)hipFree(p);
     *this = NetWork();
@@ -213,6 +214,7 @@ struct mptc_ctx {
   int16_t* d_prop = nullptr;    // intra proposals, kMaxTopK per block
   // pruned exhaustive search (prune_kernels.h), allocated on first use
   BoundTab *d_rbt = nullptr, *d_cbt = nullptr;  // bound tables of reference / current words
+  uint32_t* d_pm = nullptr;                      // passing sets of the intra windows (kPassWords per block)
   bool record = false;          // exhaustive steps record acceptance masks
   bool last_recorded = false;   // the last step did
   uint16_t* d_amask[2] = {nullptr, nullptr};  // inter, intra: 16 rows of 16 offset bits per block
@@ -321,6 +323,11 @@ void net_free(NetState& n) {
   n = NetState();
 }
 
+// Split-K weight gradients (gemm_wgrad): about kSplitGrid workgroups, so the
+// partials (splits x M x N) fit kSplitGrid tiles of 64 x 64.
+constexpr int kSplitGrid = 256;
+constexpr size_t kSplitFloats = (size_t)kSplitGrid * 64 * 64;
+
 int work_ensure(NetWork& w, int rows, bool train) {
   if (w.rows >= rows && (w.train || !train)) return MPTC_OK;
   w.release();
@@ -329,11 +336,11 @@ int work_ensure(NetWork& w, int rows, bool train) {
   if ((rc = dalloc(&w.X, r * kNetIn)) || (rc = dalloc(&w.H1, r * kNetHid)) || (rc = dalloc(&w.H2, r * kNetHid)))
     return rc;
   if (train) {
-    const size_t parts = (r + 255) / 256;
+    const size_t parts = (r + kColsumRows - 1) / kColsumRows;
     if ((rc = dalloc(&w.Y, r * kNetOut)) || (rc = dalloc(&w.dYf, r * kNetOut)) || (rc = dalloc(&w.dYb, r * kNetOut)) ||
         (rc = dalloc(&w.dH2f, r * kNetHid)) || (rc = dalloc(&w.dH2b, r * kNetHid)) ||
         (rc = dalloc(&w.dH1f, r * kNetHid)) || (rc = dalloc(&w.dH1b, r * kNetHid)) ||
-        (rc = dalloc(&w.part, parts * kNetOut)))
+        (rc = dalloc(&w.part, parts * kNetOut)) || (rc = dalloc(&w.split, kSplitFloats)))
       return rc;
   }
   w.rows = rows;
@@ -368,6 +375,30 @@ int gemm(mptc_ctx* c, hipStream_t s, int kid, int M, int N, int K, const uint16_
   return MPTC_OK;
 }
 
+// Weight gradient C (M x N, row stride ldc) = A * B with K = the batch rows:
+// few output tiles and a long K, so K is split over about kSplitGrid
+// workgroups and the partials summed in a fixed order (k_split_sum).
+int gemm_wgrad(mptc_ctx* c, hipStream_t s, NetWork& w, int M, int N, int K, const uint16_t* A, long sam, long sak,
+               const uint16_t* B, long sbk, long sbn, float* C, long ldc) {
+  const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  const int splits = std::max(1, std::min(kSplitGrid / tiles, (K + 63) / 64));
+  GemmEpi e = epi();
+  if (splits == 1 || (size_t)splits * M * N > kSplitFloats) {  // enough tiles already: no split
+    e.Cf = C;
+    e.ldc = ldc;
+    return gemm(c, s, KF_NET_FWD, M, N, K, A, sam, sak, B, sbk, sbn, e);
+  }
+  e.Cf = w.split;
+  e.ldc = N;
+  e.split_stride = (long)M * N;
+  const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64), (unsigned)splits);
+  return launch(c, s, KF_NET_FWD, [&] {
+    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, e);
+    hipLaunchKernelGGL(k_split_sum, dim3(grid_for(M * N, 256)), dim3(256), 0, s, w.split, splits, (long)M * N, M, N,
+                       C, ldc);
+  });
+}
+
 // Forward pass over w.X (rows blocks): H1, H2 in w, outputs to y (row-major
 // rows x kNetOut, or transposed kNetOut x rows when trans).
 int net_forward_q(mptc_ctx* c, hipStream_t s, const NetState& n, NetWork& w, int rows, float* y, bool trans) {
@@ -391,7 +422,7 @@ int net_forward_q(mptc_ctx* c, hipStream_t s, const NetState& n, NetWork& w, int
 }
 
 int colsum_q(mptc_ctx* c, hipStream_t s, const float* X, int rows, int cols, long ld, float* part, float* out) {
-  const int parts = (rows + 255) / 256;
+  const int parts = (rows + kColsumRows - 1) / kColsumRows;
   return launch(c, s, KF_NET_FWD, [&] {
     hipLaunchKernelGGL(k_colsum_part, dim3(grid_for(cols, 256), parts), dim3(256), 0, s, X, rows, cols, ld, part);
     hipLaunchKernelGGL(k_colsum_final, dim3(grid_for(cols, 256)), dim3(256), 0, s, part, parts, cols, out);
@@ -400,19 +431,19 @@ int colsum_q(mptc_ctx* c, hipStream_t s, const float* X, int rows, int cols, lon
 
 // Loss and gradient of a forward pass over w (rows = the batch): n.g receives
 // d(mean loss)/d(parameters), *loss (device) the mean loss.
+constexpr int kLossGrid = 256;  // workgroups of k_net_loss (one loss atomic each)
 int net_backward_q(mptc_ctx* c, hipStream_t s, NetState& n, NetWork& w, int rows, const uint32_t* target,
                    const uint8_t* eps, const int32_t* idx, float seed_w, float* loss) {
   int rc;
   if ((rc = launch(c, s, KF_NET_FWD, [&] {
-         hipLaunchKernelGGL(k_net_loss, dim3(grid_for(rows, 4)), dim3(256), 0, s, w.Y, rows, target, eps, idx, seed_w,
-                            w.dYf, w.dYb, loss);
+         hipLaunchKernelGGL(k_net_loss, dim3(std::min(grid_for(rows, 4), (unsigned)kLossGrid)), dim3(256), 0, s, w.Y, rows, target,
+                            eps, idx, seed_w, w.dYf, w.dYb, loss);
        })))
     return rc;
   GemmEpi e = epi();
   // layer 3: dW3 = H2^T dY, db3 = sum dY, dH2 = (dY W3^T) * [H2 > 0]
-  e.Cf = n.g + kOffW3;
-  e.ldc = kNetOut;
-  if ((rc = gemm(c, s, KF_NET_FWD, kNetHid, kNetOut, rows, w.H2, 1, kNetHid, w.dYb, kNetOut, 1, e))) return rc;
+  if ((rc = gemm_wgrad(c, s, w, kNetHid, kNetOut, rows, w.H2, 1, kNetHid, w.dYb, kNetOut, 1, n.g + kOffW3, kNetOut)))
+    return rc;
   if ((rc = colsum_q(c, s, w.dYf, rows, kNetOut, kNetOut, w.part, n.g + kOffB3))) return rc;
   e = epi();
   e.mask = w.H2;
@@ -424,10 +455,8 @@ int net_backward_q(mptc_ctx* c, hipStream_t s, NetState& n, NetWork& w, int rows
   if ((rc = gemm(c, s, KF_NET_FWD, rows, kNetHid, kNetOut, w.dYb, kNetOut, 1, n.wb + kOffW3, 1, kNetOut, e)))
     return rc;
   // layer 2
-  e = epi();
-  e.Cf = n.g + kOffW2;
-  e.ldc = kNetHid;
-  if ((rc = gemm(c, s, KF_NET_FWD, kNetHid, kNetHid, rows, w.H1, 1, kNetHid, w.dH2b, kNetHid, 1, e))) return rc;
+  if ((rc = gemm_wgrad(c, s, w, kNetHid, kNetHid, rows, w.H1, 1, kNetHid, w.dH2b, kNetHid, 1, n.g + kOffW2, kNetHid)))
+    return rc;
   if ((rc = colsum_q(c, s, w.dH2f, rows, kNetHid, kNetHid, w.part, n.g + kOffB2))) return rc;
   e = epi();
   e.mask = w.H1;
@@ -439,10 +468,8 @@ int net_backward_q(mptc_ctx* c, hipStream_t s, NetState& n, NetWork& w, int rows
   if ((rc = gemm(c, s, KF_NET_FWD, rows, kNetHid, kNetHid, w.dH2b, kNetHid, 1, n.wb + kOffW2, 1, kNetHid, e)))
     return rc;
   // layer 1
-  e = epi();
-  e.Cf = n.g + kOffW1;
-  e.ldc = kNetHid;
-  if ((rc = gemm(c, s, KF_NET_FWD, kNetIn, kNetHid, rows, w.X, 1, kNetIn, w.dH1b, kNetHid, 1, e))) return rc;
+  if ((rc = gemm_wgrad(c, s, w, kNetIn, kNetHid, rows, w.X, 1, kNetIn, w.dH1b, kNetHid, 1, n.g + kOffW1, kNetHid)))
+    return rc;
   return colsum_q(c, s, w.dH1f, rows, kNetHid, kNetHid, w.part, n.g + kOffB1);
 }
 
@@ -545,7 +572,8 @@ int prune_ensure(mptc_ctx* c) {
   if (c->d_rbt) return MPTC_OK;
   const size_t nb = (size_t)c->max_total;
   int rc;
-  if ((rc = dalloc(&c->d_rbt, nb)) || (rc = dalloc(&c->d_cbt, nb))) return rc;
+  if ((rc = dalloc(&c->d_rbt, nb)) || (rc = dalloc(&c->d_cbt, nb)) || (rc = dalloc(&c->d_pm, nb * kPassWords)))
+    return rc;
   return MPTC_OK;
 }
 
@@ -634,7 +662,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
     if ((rc = launch(c, s, KF_PRUNED_INTER, [&] {
            hipLaunchKernelGGL(k_pruned_search<false>, dim3(gwk256), dim3(256), 0, s, c->d_fr, c->d_tsid, c->n_wtiles,
                               tw, K, S, thr, c->d_own, c->d_need, c->d_rw, c->d_rtab, c->d_rbt, c->d_orig, nullptr,
-                              c->nbs, c->d_summ, c->d_stats);
+                              c->nbs, nullptr, c->d_summ, c->d_stats);
          })))
       return rc;
     if ((rc = launch(c, s, KF_INTER_COMBINE, [&] {
@@ -676,19 +704,19 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
   if (prune) {
     if ((rc = launch(c, s, KF_BOUND_TABLE, [&] {
            hipLaunchKernelGGL(k_bound_table_cur, dim3(grid_for(total, 256)), dim3(256), 0, s, total, c->d_cw,
-                              c->d_cbt);
+                              c->d_cbt, reinterpret_cast<uint4*>(c->d_pm));
          })))
       return rc;
     if ((rc = launch(c, s, KF_PRUNED_INTRA, [&] {
            hipLaunchKernelGGL(k_pruned_search<true>, dim3(gwk256), dim3(256), 0, s, c->d_fr, c->d_tsid, c->n_wtiles,
                               tw, K, S, thr, c->d_own, c->d_need, c->d_cw, c->d_ctab, c->d_cbt, c->d_orig, c->d_E,
-                              c->nbs, nullptr, c->d_stats);
+                              c->nbs, c->d_pm, nullptr, c->d_stats);
          })))
       return rc;
     if ((rc = launch(c, s, KF_FOLD_INTRA, [&] {
            hipLaunchKernelGGL(k_fold_intra, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
-                              total, S, thr, c->d_own, c->d_need, c->d_cw, c->d_E, c->nbs, c->d_intra, c->d_cwn,
-                              c->d_chg, c->d_stats);
+                              total, S, thr, c->d_own, c->d_need, c->d_cw, c->d_E, c->nbs, c->d_pm, c->d_intra,
+                              c->d_cwn, c->d_chg, c->d_stats);
          })))
       return rc;
     return launch(c, s, KF_APPLY, [&] {
@@ -728,18 +756,20 @@ int queue_passes(mptc_ctx* c, hipStream_t s) {
       if ((rc = lintra_pass_q(c, s, (p + 1) & 1, 0))) return rc;
     return MPTC_OK;
   }
+  // the pruned search keeps passing sets beside E (prune_kernels.h)
+  uint32_t* pm = use_prune(c) ? c->d_pm : nullptr;
   for (int p = 0; p < np; ++p) {
     const int par = p & 1;
     if ((rc = launch(c, s, KF_REEVAL, [&] {
            hipLaunchKernelGGL(k_intra_reeval, dim3(kReevalGrid), dim3(256), 0, s, c->d_fr, c->d_sid, S, thr, c->d_own,
                               c->d_need, c->d_cw, c->d_ctab, c->d_chg, c->d_E, c->nbs, c->d_dirty, c->d_orig,
-                              c->d_intra, c->d_stats, par);
+                              c->d_intra, c->d_stats, par, pm);
          })))
       return rc;
     if ((rc = launch(c, s, KF_SELECT, [&] {
            hipLaunchKernelGGL(k_intra_select, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total, S,
                               thr, c->d_own, c->d_cw, c->d_ctab, c->d_E, c->nbs, c->d_dirty, c->d_intra, c->d_chg,
-                              c->d_stats, par ^ 1, c->d_conv);
+                              c->d_stats, par ^ 1, c->d_conv, pm);
          })))
       return rc;
   }
@@ -950,7 +980,7 @@ void mptc_destroy(mptc_ctx* c) {
   c->nw.release();
   c->ew.release();
   void* lbufs[] = {c->d_h2p, c->d_pinter, c->d_prop, c->d_seen, c->d_errs, c->d_amask[0], c->d_amask[1],
-                   c->d_rbt,  c->d_cbt};
+                   c->d_rbt,  c->d_cbt,  c->d_pm};
   for (void* p : lbufs)
     if (p) (void)hipFree(p);
   if (c->h_stats) (void)hipHostFree(c->h_stats);

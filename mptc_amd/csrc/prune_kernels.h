@@ -37,6 +37,7 @@ namespace mptc {
 
 constexpr int16_t kPruned = 32766;  // E value of a candidate proved not to pass
 constexpr int kPruneMaxS = 8;       // window positions of one batch fit the LDS list
+static_assert(4 * kPruneMaxS * kPruneMaxS <= 32 * kPassWords, "a passing set holds every window position");
 
 // Per index word: byte masks of groups 0..2 over the four 4-pixel chunks and
 // 1 / (pixels per group) for groups 0..3 (0 for an empty group).
@@ -193,8 +194,8 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
                                                        const uint32_t* __restrict__ words,
                                                        const float4* __restrict__ tabs,
                                                        const BoundTab* __restrict__ btab, const int* __restrict__ orig,
-                                                       uint2* __restrict__ E4, size_t nbs, uint4* __restrict__ summ,
-                                                       DevStats* __restrict__ st) {
+                                                       uint2* __restrict__ E4, size_t nbs, uint32_t* __restrict__ pm,
+                                                       uint4* __restrict__ summ, DevStats* __restrict__ st) {
   __shared__ uint8_t s_snap[512];
   __shared__ float2 s_ab[4];
   __shared__ uint32_t s_blk[4][kPackedBlk][64];      // [wave][field][lane]
@@ -317,12 +318,23 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (act && kIntra) {  // intra: E, for the fixpoint passes
+    if (act && kIntra) {  // intra: E and the passing set, for the fixpoint passes
+      uint32_t bits = 0;  // bit k: position pb + k passes
       for (int k = 0; k < nb; k += 4) {
-        const uint32_t lo = (uint32_t)(uint16_t)s_e[wave][k][lane] | ((uint32_t)(uint16_t)s_e[wave][k + 1][lane] << 16);
-        const uint32_t hi =
-            (uint32_t)(uint16_t)s_e[wave][k + 2][lane] | ((uint32_t)(uint16_t)s_e[wave][k + 3][lane] << 16);
+        const int16_t e0 = s_e[wave][k][lane], e1 = s_e[wave][k + 1][lane], e2 = s_e[wave][k + 2][lane],
+                      e3 = s_e[wave][k + 3][lane];
+        const uint32_t lo = (uint32_t)(uint16_t)e0 | ((uint32_t)(uint16_t)e1 << 16);
+        const uint32_t hi = (uint32_t)(uint16_t)e2 | ((uint32_t)(uint16_t)e3 << 16);
         E4[(size_t)((pb + k) >> 2) * nbs + g] = make_uint2(lo, hi);
+        bits |= (uint32_t)((e0 != kNoCand && e0 <= thr) | (e1 != kNoCand && e1 <= thr) << 1 |
+                           (e2 != kNoCand && e2 <= thr) << 2 | (e3 != kNoCand && e3 <= thr) << 3)
+                << k;
+      }
+      if (bits) {  // rare; the batch's 16 bits may straddle two words, which other chunks' waves share
+        const int sh = pb & 31;
+        uint32_t* wp = pm + (size_t)g * kPassWords + (pb >> 5);
+        atomicOr(wp, bits << sh);
+        if (sh > 16 && (bits >> (32 - sh)) != 0u) atomicOr(wp + 1, bits >> (32 - sh));
       }
     } else if (!kIntra && __ballot(act && s_pass[wave][lane] != 0) != 0ull) {
       // inter: fold the batch's passing deltas in scan order.  The selection
@@ -367,52 +379,14 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
   wg_count(&s_acc, n_exact, kIntra ? st->exact_intra : st->exact_inter);
 }
 
-// The selection rule (codec/dxt_image.cpp:697-708) over the passing deltas of
-// block g's E (for thr >= 0 that is the full rule's result; see above).  The
-// S^2 groups of four are loaded 32 at a time.
-__device__ __forceinline__ Fold fold_passing(const uint2* __restrict__ E4, size_t nbs, int g, int S, int thr) {
-  const int W2 = 2 * S, groups = S * S;
-  Fold fd;
-  int row = -1;
-  for (int g0 = 0; g0 < groups; g0 += 32) {
-    uint2 buf[32];
-#pragma unroll
-    for (int b = 0; b < 32; ++b)  // past the last group: absent candidates
-      buf[b] = g0 + b < groups ? E4[(size_t)(g0 + b) * nbs + g] : make_uint2(0x7FFF7FFFu, 0x7FFF7FFFu);
-#pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      const uint2 gg = buf[b];
-      bool any = false;
-#pragma unroll
-      for (int nn = 0; nn < 4; ++nn) {
-        const int e = e_at(gg, nn);
-        any = any || (e != kNoCand && e <= thr);
-      }
-      if (!any) continue;
-#pragma unroll
-      for (int nn = 0; nn < 4; ++nn) {
-        const int e = e_at(gg, nn);
-        if (e == kNoCand || e > thr) continue;
-        const int pos = 4 * (g0 + b) + nn;
-        const int r = pos / W2;
-        if (r != row) {
-          row = r;
-          fd.row_done = false;
-        }
-        fd.step(e, pos);
-      }
-    }
-  }
-  return fd;
-}
-
-// Intra decisions from E (as k_intra_combine): dec, the new current word and
-// the first changed-word list.
+// Intra decisions from the passing sets and E (as k_intra_combine): dec, the
+// new current word and the first changed-word list.
 __global__ void __launch_bounds__(256) k_fold_intra(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
                                                     int total, int S, int thr, const uint64_t* __restrict__ own,
                                                     const uint8_t* __restrict__ need,
                                                     const uint32_t* __restrict__ words,
                                                     const uint2* __restrict__ E4, size_t nbs,
+                                                    const uint32_t* __restrict__ pm,
                                                     uint64_t* __restrict__ dec, uint32_t* __restrict__ cw_new,
                                                     int* __restrict__ chg_list, DevStats* __restrict__ st) {
   __shared__ unsigned int s_base;
@@ -421,7 +395,7 @@ __global__ void __launch_bounds__(256) k_fold_intra(const FrameDev* __restrict__
   if (g < total && need[g]) {
     const FrameDev& d = fr[sid[g]];
     const int k = g - d.base, bx = k % d.bw, by = k / d.bw;
-    const Fold fd = fold_passing(E4, nbs, g, S, thr);
+    const Fold fd = fold_passing_set(pm, E4, nbs, g, S);
     const uint64_t dc = intra_decision(words, d.base, d.bw, bx, by, S, thr, fd);
     dec[g] = dc;
     const uint32_t w = dec_acc(dc) ? dec_word(dc) : (uint32_t)(own[g] >> 32);
@@ -443,11 +417,15 @@ __global__ void __launch_bounds__(256) k_bound_table_ref(const FrameDev* __restr
   rbt[g] = bound_table(rw[g]);
 }
 
-// Bound tables of the current words at the start of the intra search.
+// Bound tables of the current words at the start of the intra search; the
+// passing sets start empty.
 __global__ void __launch_bounds__(256) k_bound_table_cur(int total, const uint32_t* __restrict__ cw,
-                                                         BoundTab* __restrict__ cbt) {
+                                                         BoundTab* __restrict__ cbt, uint4* __restrict__ pm4) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < total) cbt[g] = bound_table(cw[g]);
+  if (g >= total) return;
+  cbt[g] = bound_table(cw[g]);
+  pm4[2 * g] = make_uint4(0u, 0u, 0u, 0u);
+  pm4[2 * g + 1] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 }  // namespace mptc
