@@ -449,6 +449,7 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
     const bool act = inside && need[g];
     uint32_t cur = inside ? cw[g] : 0u;
     const uint32_t start = cur;
+    const uint32_t own_w = act ? (uint32_t)(own[g] >> 32) : 0u;  // loaded once, not per decision
     int src[K];  // lane holding the proposal's block, or -1 (other tile / none)
     uint32_t w[K], sn[K];
     int er[K], Ls[K];
@@ -485,7 +486,7 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
           bi = i;
         }
       dc = 0;
-      uint32_t nw = (uint32_t)(own[g] >> 32);
+      uint32_t nw = own_w;
 #pragma unroll
       for (int i = 0; i < K; ++i)
         if (i == bi && best <= thr) {

@@ -230,25 +230,6 @@ __device__ __forceinline__ void admitted_words(const WinMask& m, uint32_t (&w)[8
   }
 }
 
-// Sorts K keys in descending order (bitonic network, branch-free).
-template <int K>
-__device__ __forceinline__ void bitonic_desc(uint32_t (&a)[K]) {
-#pragma unroll
-  for (int k = 2; k <= K; k <<= 1)
-#pragma unroll
-    for (int j = k >> 1; j >= 1; j >>= 1)
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        const int l = i ^ j;
-        if (l > i) {
-          const uint32_t x = a[i], y = a[l];
-          const bool down = (i & k) == 0;
-          a[i] = down ? max(x, y) : min(x, y);
-          a[l] = down ? min(x, y) : max(x, y);
-        }
-      }
-}
-
 // top := the K largest of top and b, both sorted in descending order: the
 // elementwise maximum of top and reversed b is bitonic and holds them; a
 // bitonic merge sorts it.
@@ -271,16 +252,22 @@ __device__ __forceinline__ void merge_top(uint32_t (&top)[K], const uint32_t (&b
 
 __device__ __forceinline__ int key_offset(uint32_t key) { return key ? 255 - (int)(key & 0xFFu) : -1; }
 
-// Inserts a key into a list sorted in descending order (branch-free: every
-// lane of the wave does the same work, whatever its data).
+// Inserts a key into a list sorted in descending order: entry i becomes the
+// median of t[i - 1], t[i] and the key (t[i - 1] >= t[i], so the key clamped
+// into [t[i], t[i - 1]]), entry 0 the maximum of t[0] and the key.  Every
+// entry depends only on the old list: K independent v_max_u32 / v_med3_u32,
+// branch-free.
+__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+  return max(min(a, b), min(max(a, b), c));  // selected as v_med3_u32
+}
 template <int K>
 __device__ __forceinline__ void key_insert(uint32_t (&t)[K], uint32_t key) {
+  uint32_t n[K];
+  n[0] = max(t[0], key);
 #pragma unroll
-  for (int i = 0; i < K; ++i) {
-    const uint32_t hi = max(t[i], key);
-    key = min(t[i], key);
-    t[i] = hi;
-  }
+  for (int i = 1; i < K; ++i) n[i] = med3_u32(t[i - 1], t[i], key);
+#pragma unroll
+  for (int i = 0; i < K; ++i) t[i] = n[i];
 }
 
 // Layer 3.  With Y non-null: writes the 262 outputs of each row (row-major).
@@ -332,9 +319,8 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
       }
       continue;
     }
-    // fused top-k: each lane ranks its 128 logits in chunks of K (sorted by
-    // a bitonic network, merged into the running top K); lane h = 0 merges
-    // the partner lane's (l + 32) list and writes
+    // fused top-k: each lane inserts its 128 logits' keys into a running top
+    // K; lane h = 0 merges the partner lane's (l + 32) list and writes
     bool act = row < rows;
     uint32_t vw[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
     if (act) {
@@ -349,17 +335,10 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb)
 #pragma unroll
-      for (int c0 = 0; c0 < 16; c0 += K) {
-        uint32_t chunk[K];
-#pragma unroll
-        for (int t = 0; t < K; ++t) {
-          const int i = c0 + t;
-          const int bit = 8 * (i >> 2) + 4 * h + (i & 3);  // L = nb * 32 + bit
-          const uint32_t ok = (uint32_t)((int32_t)(vw[nb] << (31 - bit)) >> 31);
-          chunk[t] = rank_key(acc[nb][i], nb * 32 + bit) & ok;
-        }
-        bitonic_desc<K>(chunk);
-        merge_top<K>(top, chunk);
+      for (int i = 0; i < 16; ++i) {
+        const int bit = 8 * (i >> 2) + 4 * h + (i & 3);  // L = nb * 32 + bit
+        const uint32_t ok = (uint32_t)((int32_t)(vw[nb] << (31 - bit)) >> 31);
+        key_insert<K>(top, rank_key(acc[nb][i], nb * 32 + bit) & ok);
       }
     uint32_t partner[K];
 #pragma unroll
