@@ -585,7 +585,8 @@ int infer_q(mptc_ctx* c, hipStream_t s, const NetState& n, const uint16_t* X, in
   const unsigned grid = (unsigned)std::max(1, std::min(c->n_cu, (n_tiles + kInferWaves - 1) / kInferWaves));
   const int S = c->cfg.search_area;
   int rc = launch(c, s, KF_NET_L12, [&] {
-    hipLaunchKernelGGL(k_net_l12, dim3(grid), dim3(64 * kInferWaves), 0, s, X, rows, n.pk, h2p);
+    const unsigned g12 = (unsigned)std::max(1, std::min(c->n_cu, (n_tiles + kL12Waves - 1) / kL12Waves));
+    hipLaunchKernelGGL(k_net_l12, dim3(g12), dim3(64 * kL12Waves), 0, s, X, rows, n.pk, h2p);
   });
   if (rc) return rc;
   return launch(c, s, KF_NET_L3, [&] {

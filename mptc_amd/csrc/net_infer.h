@@ -117,12 +117,13 @@ __device__ __forceinline__ f32x16_t bias_tile(const float* __restrict__ sb, int 
   return a;
 }
 
-constexpr int kInferWaves = 8;  // waves per workgroup: two per SIMD share the LDS weights
+constexpr int kInferWaves = 8;  // waves per workgroup of k_net_l3 (255 VGPRs: two per SIMD)
+constexpr int kL12Waves = 16;   // waves per workgroup of k_net_l12 (<= 128 VGPRs: four per SIMD)
 
 // Layers 1 and 2.  X: rows x 48 bf16; H2p: per 32-block tile, per k-step s,
 // per lane, the 8 bf16 layer-3 operand values (kH2TileBytes per tile).
 // One workgroup per CU; each wave loops over tiles.
-__global__ void __launch_bounds__(64 * kInferWaves) k_net_l12(const uint16_t* __restrict__ X, int rows,
+__global__ void __launch_bounds__(64 * kL12Waves) k_net_l12(const uint16_t* __restrict__ X, int rows,
                                                  const uint16_t* __restrict__ pk, uint4* __restrict__ H2p) {
   __shared__ uint4 sW1[kNetHid * kNetIn / 8];   // 24 KiB, rows of 6 chunks
   __shared__ uint4 sW2[kNetHid * kNetHid / 8];  // 128 KiB, rows of 32 chunks, swizzled
@@ -134,8 +135,8 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l12(const uint16_t* __
   __syncthreads();
   const int lane = threadIdx.x & 63, m = lane & 31, h = lane >> 5;
   const int n_tiles = (rows + 31) / 32;
-  const int wave_g = blockIdx.x * kInferWaves + (threadIdx.x >> 6);
-  for (int tile = wave_g; tile < n_tiles; tile += gridDim.x * kInferWaves) {
+  const int wave_g = blockIdx.x * kL12Waves + (threadIdx.x >> 6);
+  for (int tile = wave_g; tile < n_tiles; tile += gridDim.x * kL12Waves) {
     const int row = tile * 32 + m;
     // layer 1: B operands straight from X (k = 16s + 8h .. +7)
     bf16x8_t xb[3];
