@@ -639,6 +639,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
                       (!c->net[MPTC_NET_INTER].use_prior ||
                        (c->mode == MPTC_SEARCH_LEARNED && !c->net[MPTC_NET_INTRA].use_prior));
   bool x_done = want_x;
+  bool prep_done = false;  // need and the starting intra words already set
   if ((rc = launch(c, s, KF_FIT, [&] {
          const int n_st = (int)(sizeof(DevStats) / 4);
          hipLaunchKernelGGL(k_block_prep, dim3(grid_for(std::max(total, n_st + c->n_tiles), wg)), dim3(wg), 0, s,
@@ -685,13 +686,17 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
       return rc;
   } else {
     if ((rc = proposals_q(c, s, MPTC_NET_INTER, nullptr, c->d_pinter, x_done))) return rc;
+    // the learned intra search needs only k_intra_prep's flags and starting
+    // words, which the inter evaluation sets for every block of the batch
+    prep_done = c->mode == MPTC_SEARCH_LEARNED;
     if ((rc = launch(c, s, KF_PROPOSE, [&] {
            hipLaunchKernelGGL(k_eval_inter, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total, S,
-                              thr, c->top_k, c->d_pinter, c->d_own, c->d_rw, c->d_rtab, c->d_inter, c->d_stats);
+                              thr, c->top_k, c->d_pinter, c->d_own, c->d_rw, c->d_rtab, c->d_inter,
+                              prep_done ? c->d_need : nullptr, prep_done ? c->d_cw : nullptr, c->d_stats);
          })))
       return rc;
   }
-  if ((rc = launch(c, s, KF_PREP, [&] {
+  if (!prep_done && (rc = launch(c, s, KF_PREP, [&] {
          const bool lrn = c->mode == MPTC_SEARCH_LEARNED;  // its intra passes use no tables, errors or flags
          hipLaunchKernelGGL(k_intra_prep, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
                             c->d_own, c->d_inter, c->d_need, c->d_cw, lrn ? nullptr : c->d_ctab,

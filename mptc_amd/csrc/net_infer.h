@@ -379,12 +379,15 @@ __global__ void __launch_bounds__(256) k_top_scores(const FrameDev* __restrict__
 // Learned inter search: each proposed offset is evaluated exactly (the same
 // re-fit and error as InterBlockSearch, codec/dxt_image.cpp:715-774); the
 // valid one with the smallest error delta (first in rank order on ties) is
-// reused if the delta passes the threshold (:890).
+// reused if the delta passes the threshold (:890).  With need and cw given
+// (the learned intra search follows), it also sets them for every block of
+// the batch as k_intra_prep does, which then is not launched.
 __global__ void __launch_bounds__(256) k_eval_inter(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
                                                     int total, int S, int thr, int kk,
                                                     const int16_t* __restrict__ prop,
                                                     const uint64_t* __restrict__ own, const uint32_t* __restrict__ rw,
                                                     const float4* __restrict__ rtab, uint64_t* __restrict__ dec_out,
+                                                    uint8_t* __restrict__ need, uint32_t* __restrict__ cw,
                                                     DevStats* __restrict__ st) {
   __shared__ uint8_t s_snap[512];
   __shared__ float2 s_ab[4];
@@ -392,6 +395,10 @@ __global__ void __launch_bounds__(256) k_eval_inter(const FrameDev* __restrict__
   load_lds_tables(s_snap, s_ab);
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long n_cand = 0;
+  if (need && g < total && fr[sid[g]].is_intra) {  // (as k_intra_prep) an intra frame's block searches
+    need[g] = 1;
+    cw[g] = (uint32_t)(own[g] >> 32);
+  }
   if (g < total && !fr[sid[g]].is_intra) {
     const FrameDev& d = fr[sid[g]];
     const int k = g - d.base, bx = k % d.bw, by = k / d.bw, bw = d.bw;
@@ -422,6 +429,10 @@ __global__ void __launch_bounds__(256) k_eval_inter(const FrameDev* __restrict__
       dc = pack_dec(best_c, (uint32_t)(best_L % kNetWin - 8 + S) | 0x80u, (uint32_t)(best_L / kNetWin - 8 + S) | 0x80u,
                     true);
     dec_out[g] = dc;
+    if (need) {  // (as k_intra_prep) a block without an inter reuse searches the intra window
+      need[g] = dec_acc(dc) ? 0 : 1;
+      cw[g] = dec_acc(dc) ? dec_word(dc) : (uint32_t)(own[g] >> 32);
+    }
   }
   wg_count(&s_acc, n_cand, st->inter_cands);
 }
