@@ -8,6 +8,7 @@
 //                  candidate tables of the reference frames' index words
 //   k_inter        InterBlockSearch + accept test (lane per block, wave tile)
 //   k_intra_prep   which blocks need the intra search; initial current words
+//                  (pruned search: their bound tables, empty passing sets)
 //   k_intra_dense  every intra candidate of every such block: stores the error
 //                  deltas E, folds the selection, lists the words that change
 //   k_intra_apply  the listed words become current (first pass only)
@@ -66,6 +67,36 @@ struct DevStats {
   unsigned int n_changed[2];             // length of the changed-word list, by pass parity
   unsigned int pass_done;                // workgroups of the running fixpoint pass that finished
 };
+
+// The pruning bound's table of an index word (prune_kernels.h).
+// Per index word: byte masks of groups 0..2 over the four 4-pixel chunks and
+// 1 / (pixels per group) for groups 0..3 (0 for an empty group).
+struct BoundTab {
+  uint4 m0, m1, m2;
+  float4 inv;
+};
+
+__device__ __forceinline__ BoundTab bound_table(uint32_t c) {
+  uint32_t m[3][4];
+  uint32_t n[3] = {0u, 0u, 0u};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t x = spread_indices(c, j);
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      m[g][j] = byte_gather(1u << (8 * g), x);  // 1 in byte n where pixel 4j+n has index g
+      n[g] = __builtin_amdgcn_udot4(m[g][j], 0x01010101u, n[g], false);
+    }
+  }
+  const uint32_t n3 = 16u - n[0] - n[1] - n[2];
+  BoundTab t;
+  t.m0 = make_uint4(m[0][0], m[0][1], m[0][2], m[0][3]);
+  t.m1 = make_uint4(m[1][0], m[1][1], m[1][2], m[1][3]);
+  t.m2 = make_uint4(m[2][0], m[2][1], m[2][2], m[2][3]);
+  t.inv = make_float4(n[0] ? 1.0f / (float)n[0] : 0.0f, n[1] ? 1.0f / (float)n[1] : 0.0f,
+                      n[2] ? 1.0f / (float)n[2] : 0.0f, n3 ? 1.0f / (float)n3 : 0.0f);
+  return t;
+}
 
 // Per-block search decision, packed into 64 bits:
 // [0,32) index word, [32,40) motion x, [40,48) motion y, bit 48 accepted.
@@ -338,7 +369,8 @@ __global__ void __launch_bounds__(256) k_inter_combine(const FrameDev* __restric
 __global__ void __launch_bounds__(256) k_intra_prep(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid, int total,
                              const uint64_t* __restrict__ own, const uint64_t* __restrict__ inter_dec,
                              uint8_t* __restrict__ need, uint32_t* __restrict__ cw, float4* __restrict__ ctab,
-                             uint32_t* __restrict__ dirty, int* __restrict__ orig) {
+                             uint32_t* __restrict__ dirty, int* __restrict__ orig, BoundTab* __restrict__ cbt,
+                             uint4* __restrict__ pm4) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const FrameDev& d = fr[sid[g]];
@@ -359,6 +391,11 @@ __global__ void __launch_bounds__(256) k_intra_prep(const FrameDev* __restrict__
   dirty[g] = 0;
   const CandTab t = cand_table(w);
   ctab[g] = make_float4(t.asq, t.bsq, t.ab, t.f);
+  if (cbt) {  // the pruned intra search: bound table of the word, passing set empty
+    cbt[g] = bound_table(w);
+    pm4[2 * g] = make_uint4(0u, 0u, 0u, 0u);
+    pm4[2 * g + 1] = make_uint4(0u, 0u, 0u, 0u);
+  }
   BlockPx px;
   load_block(d.rgb, d.width, k % d.bw, k / d.bw, px);
   orig[g] = own_error(px, o);

@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256) k_block_prep(const FrameDev* __restrict__
                                                     int total, uint64_t* __restrict__ own, uint32_t* __restrict__ rw,
                                                     float4* __restrict__ rtab, uint16_t* __restrict__ X,
                                                     uint32_t* __restrict__ z1, int n1, uint32_t* __restrict__ z2,
-                                                    int n2) {
+                                                    int n2, BoundTab* __restrict__ rbt) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < n1) z1[g] = 0u;
   else if (g - n1 < n2) z2[g - n1] = 0u;
@@ -381,6 +381,7 @@ __global__ void __launch_bounds__(256) k_block_prep(const FrameDev* __restrict__
     const CandTab t = cand_table(c);
     rw[g] = c;
     rtab[g] = make_float4(t.asq, t.bsq, t.ab, t.f);
+    if (rbt) rbt[g] = bound_table(c);  // the pruned inter search's bound table
   }
   if (!own && !X) return;
   BlockPx px;

@@ -104,15 +104,14 @@ int init_tables(int device) {
 enum KernelId {
   KF_FIT, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
   KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
-  KF_NET_L3, KF_BOUND_TABLE, KF_PRUNED_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
+  KF_NET_L3, KF_PRUNED_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
 };
 const char* kKernelNames[KF_N] = {"fit",          "inter",
                                   "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
                                   "intra_apply",  "intra_reeval", "intra_select", "commit",
                                   "unique",       "net_input",   "net_forward", "propose_inter",
                                   "top_intra",    "lintra_pass",  "net_l12",     "net_l3",
-                                  "bound_table",  "pruned_inter", "pruned_intra",
-                                  "fold_intra"};
+                                  "pruned_inter", "pruned_intra", "fold_intra"};
 constexpr int kMaxChunks = 8;  // window-row chunks per block in the search kernels
 
 constexpr int kReevalGrid = 4096;  // workgroups of the re-evaluation pass (grid-stride)
@@ -640,27 +639,23 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
                        (c->mode == MPTC_SEARCH_LEARNED && !c->net[MPTC_NET_INTRA].use_prior));
   bool x_done = want_x;
   bool prep_done = false;  // need and the starting intra words already set
+  const bool prune = use_prune(c);
   if ((rc = launch(c, s, KF_FIT, [&] {
          const int n_st = (int)(sizeof(DevStats) / 4);
          hipLaunchKernelGGL(k_block_prep, dim3(grid_for(std::max(total, n_st + c->n_tiles), wg)), dim3(wg), 0, s,
                             c->d_fr, c->d_sid, total, fit ? c->d_own : nullptr, c->d_rw, c->d_rtab,
                             want_x ? c->ew.X : nullptr, reinterpret_cast<uint32_t*>(c->d_stats), n_st, c->d_tiles,
-                            c->n_tiles);
+                            c->n_tiles, prune ? c->d_rbt : nullptr);
        })))
     return rc;
   const int K = chunks_for(S, c->tune.win_chunks);
   const unsigned gwk = grid_for(c->n_wtiles * K * 64, wg);
-  const bool prune = use_prune(c);
   const unsigned gwk256 = grid_for(c->n_wtiles * K * 64, 256);
   if (!c->any_inter) {
     // a batch of intra frames only: no inter search (k_intra_prep reads no inter decision)
   } else if (prune) {
-    // inter: bound, exact evaluation of the survivors, fold (prune_kernels.h)
-    if ((rc = launch(c, s, KF_BOUND_TABLE, [&] {
-           hipLaunchKernelGGL(k_bound_table_ref, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
-                              total, c->d_rw, c->d_rbt);
-         })))
-      return rc;
+    // inter: bound (tables from k_block_prep), exact evaluation of the
+    // survivors, fold (prune_kernels.h)
     if ((rc = launch(c, s, KF_PRUNED_INTER, [&] {
            hipLaunchKernelGGL(k_pruned_search<false>, dim3(gwk256), dim3(256), 0, s, c->d_fr, c->d_tsid, c->n_wtiles,
                               tw, K, S, thr, c->d_own, c->d_need, c->d_rw, c->d_rtab, c->d_rbt, c->d_orig, nullptr,
@@ -700,19 +695,15 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
          const bool lrn = c->mode == MPTC_SEARCH_LEARNED;  // its intra passes use no tables, errors or flags
          hipLaunchKernelGGL(k_intra_prep, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
                             c->d_own, c->d_inter, c->d_need, c->d_cw, lrn ? nullptr : c->d_ctab,
-                            lrn ? nullptr : c->d_dirty, lrn ? nullptr : c->d_orig);
+                            lrn ? nullptr : c->d_dirty, lrn ? nullptr : c->d_orig, prune ? c->d_cbt : nullptr,
+                            prune ? reinterpret_cast<uint4*>(c->d_pm) : nullptr);
        })))
     return rc;
   if (c->mode == MPTC_SEARCH_LEARNED) {
     if ((rc = proposals_q(c, s, MPTC_NET_INTRA, c->d_need, c->d_prop, x_done))) return rc;
     return lintra_pass_q(c, s, 0, 1);
   }
-  if (prune) {
-    if ((rc = launch(c, s, KF_BOUND_TABLE, [&] {
-           hipLaunchKernelGGL(k_bound_table_cur, dim3(grid_for(total, 256)), dim3(256), 0, s, total, c->d_cw,
-                              c->d_cbt, reinterpret_cast<uint4*>(c->d_pm));
-         })))
-      return rc;
+  if (prune) {  // bound tables and cleared passing sets from k_intra_prep
     if ((rc = launch(c, s, KF_PRUNED_INTRA, [&] {
            hipLaunchKernelGGL(k_pruned_search<true>, dim3(gwk256), dim3(256), 0, s, c->d_fr, c->d_tsid, c->n_wtiles,
                               tw, K, S, thr, c->d_own, c->d_need, c->d_cw, c->d_ctab, c->d_cbt, c->d_orig, c->d_E,

@@ -26,6 +26,10 @@
 //   k_fold_intra    per block, the reference's selection over the passing
 //                   deltas of E: the intra decision with the changed-word list
 //                   (as k_intra_combine)
+//
+// The words' bound tables (BoundTab, batch_kernels.h) are built by
+// k_block_prep (the previous frame's words) and k_intra_prep (the current
+// words, which also clears the passing sets).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,35 +42,6 @@ namespace mptc {
 constexpr int16_t kPruned = 32766;  // E value of a candidate proved not to pass
 constexpr int kPruneMaxS = 8;       // window positions of one batch fit the LDS list
 static_assert(4 * kPruneMaxS * kPruneMaxS <= 32 * kPassWords, "a passing set holds every window position");
-
-// Per index word: byte masks of groups 0..2 over the four 4-pixel chunks and
-// 1 / (pixels per group) for groups 0..3 (0 for an empty group).
-struct BoundTab {
-  uint4 m0, m1, m2;
-  float4 inv;
-};
-
-__device__ __forceinline__ BoundTab bound_table(uint32_t c) {
-  uint32_t m[3][4];
-  uint32_t n[3] = {0u, 0u, 0u};
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const uint32_t x = spread_indices(c, j);
-#pragma unroll
-    for (int g = 0; g < 3; ++g) {
-      m[g][j] = byte_gather(1u << (8 * g), x);  // 1 in byte n where pixel 4j+n has index g
-      n[g] = __builtin_amdgcn_udot4(m[g][j], 0x01010101u, n[g], false);
-    }
-  }
-  const uint32_t n3 = 16u - n[0] - n[1] - n[2];
-  BoundTab t;
-  t.m0 = make_uint4(m[0][0], m[0][1], m[0][2], m[0][3]);
-  t.m1 = make_uint4(m[1][0], m[1][1], m[1][2], m[1][3]);
-  t.m2 = make_uint4(m[2][0], m[2][1], m[2][2], m[2][3]);
-  t.inv = make_float4(n[0] ? 1.0f / (float)n[0] : 0.0f, n[1] ? 1.0f / (float)n[1] : 0.0f,
-                      n[2] ? 1.0f / (float)n[2] : 0.0f, n3 ? 1.0f / (float)n3 : 0.0f);
-  return t;
-}
 
 // Pixel side of the bound for one block.
 struct BoundBlk {
@@ -405,27 +380,6 @@ __global__ void __launch_bounds__(256) k_fold_intra(const FrameDev* __restrict__
     }
   }
   wg_append(changed, g, chg_list, &st->n_changed[0], &s_base);
-}
-
-// Bound tables of the previous frames' words (inter candidates).
-__global__ void __launch_bounds__(256) k_bound_table_ref(const FrameDev* __restrict__ fr,
-                                                         const uint8_t* __restrict__ sid, int total,
-                                                         const uint32_t* __restrict__ rw,
-                                                         BoundTab* __restrict__ rbt) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total || fr[sid[g]].is_intra) return;
-  rbt[g] = bound_table(rw[g]);
-}
-
-// Bound tables of the current words at the start of the intra search; the
-// passing sets start empty.
-__global__ void __launch_bounds__(256) k_bound_table_cur(int total, const uint32_t* __restrict__ cw,
-                                                         BoundTab* __restrict__ cbt, uint4* __restrict__ pm4) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  cbt[g] = bound_table(cw[g]);
-  pm4[2 * g] = make_uint4(0u, 0u, 0u, 0u);
-  pm4[2 * g + 1] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 }  // namespace mptc
