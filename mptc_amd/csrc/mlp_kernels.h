@@ -225,7 +225,8 @@ __global__ void __launch_bounds__(256) k_net_loss(const float* __restrict__ Y, i
 }
 
 // Column sums in two deterministic stages: part[p][c] = sum of rows
-// p*kColsumRows .. (p+1)*kColsumRows - 1, then out[c] = sum over p in order.
+// p*kColsumRows .. (p+1)*kColsumRows - 1, then out[c] = sum over p in a fixed
+// order (k_colsum_final).
 constexpr int kColsumRows = 32;  // a 4096-row batch: 128 partials per column
 __global__ void __launch_bounds__(256) k_colsum_part(const float* __restrict__ X, int rows, int cols, long ld,
                                                      float* __restrict__ part) {
@@ -237,13 +238,17 @@ __global__ void __launch_bounds__(256) k_colsum_part(const float* __restrict__ X
   part[(long)p * cols + c] = s;
 }
 
+// out[c] = sum over p of part[p][c]: a wave per column; lane l adds parts
+// l, l + 64, ... in order, then the wave adds its lanes in a fixed tree (the
+// same order every run).
 __global__ void __launch_bounds__(256) k_colsum_final(const float* __restrict__ part, int parts, int cols,
                                                       float* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (c >= cols) return;  // wave-uniform
   float s = 0.0f;
-  for (int p = 0; p < parts; ++p) s += part[(long)p * cols + c];
-  out[c] = s;
+  for (int p = lane; p < parts; p += 64) s += part[(long)p * cols + c];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) out[c] = s;
 }
 
 // Adam on the fp32 master parameters; refreshes their bf16 copy.

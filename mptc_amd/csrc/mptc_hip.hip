@@ -424,7 +424,7 @@ int colsum_q(mptc_ctx* c, hipStream_t s, const float* X, int rows, int cols, lon
   const int parts = (rows + kColsumRows - 1) / kColsumRows;
   return launch(c, s, KF_NET_FWD, [&] {
     hipLaunchKernelGGL(k_colsum_part, dim3(grid_for(cols, 256), parts), dim3(256), 0, s, X, rows, cols, ld, part);
-    hipLaunchKernelGGL(k_colsum_final, dim3(grid_for(cols, 256)), dim3(256), 0, s, part, parts, cols, out);
+    hipLaunchKernelGGL(k_colsum_final, dim3(grid_for(cols, 4)), dim3(256), 0, s, part, parts, cols, out);  // a wave per column
   });
 }
 
