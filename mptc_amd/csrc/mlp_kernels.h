@@ -98,17 +98,49 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const ui
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
   const int kspan = ((K + (int)gridDim.z - 1) / (int)gridDim.z + 31) & ~31;
   const int kb = (int)blockIdx.z * kspan, ke = min(K, kb + kspan);
+  // 16-byte loads of 8 elements along an operand's contiguous dimension where
+  // the other stride keeps them aligned (k0, m0, n0 are multiples of 8)
+  const bool a16 = ((uintptr_t)A & 15u) == 0, b16 = ((uintptr_t)B & 15u) == 0;
+  const bool a_k = a16 && sak == 1 && (sam & 7) == 0, a_m = a16 && sam == 1 && (sak & 7) == 0;
+  const bool b_k = b16 && sbk == 1 && (sbn & 7) == 0, b_n = b16 && sbn == 1 && (sbk & 7) == 0;
   for (int k0 = kb; k0 < ke; k0 += 32) {
-    // consecutive threads walk the contiguous dimension of each operand
-    for (int e = tid; e < 64 * 32; e += 256) {
-      const int mm = sak == 1 ? e / 32 : e % 64, kk = sak == 1 ? e % 32 : e / 64;
-      const int m = m0 + mm, k = k0 + kk;
-      sA[mm][kk] = (m < M && k < ke) ? A[(long)m * sam + (long)k * sak] : (uint16_t)0;
+    const bool kfull = k0 + 32 <= ke;
+    if (kfull && m0 + 64 <= M && a_k) {  // a row's 8 consecutive k
+      const int mm = tid >> 2, kq = (tid & 3) * 8;
+      *reinterpret_cast<uint4*>(&sA[mm][kq]) =
+          *reinterpret_cast<const uint4*>(A + (long)(m0 + mm) * sam + (k0 + kq));
+    } else if (kfull && m0 + 64 <= M && a_m) {  // 8 consecutive m at one k
+      const int kk = tid >> 3, mq = (tid & 7) * 8;
+      const uint4 v = *reinterpret_cast<const uint4*>(A + (long)(m0 + mq) + (long)(k0 + kk) * sak);
+      uint16_t h[8];
+      __builtin_memcpy(h, &v, 16);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sA[mq + j][kk] = h[j];
+    } else {
+      // consecutive threads walk the contiguous dimension of each operand
+      for (int e = tid; e < 64 * 32; e += 256) {
+        const int mm = sak == 1 ? e / 32 : e % 64, kk = sak == 1 ? e % 32 : e / 64;
+        const int m = m0 + mm, k = k0 + kk;
+        sA[mm][kk] = (m < M && k < ke) ? A[(long)m * sam + (long)k * sak] : (uint16_t)0;
+      }
     }
-    for (int e = tid; e < 64 * 32; e += 256) {
-      const int nn = sbn == 1 ? e % 64 : e / 32, kk = sbn == 1 ? e / 64 : e % 32;
-      const int k = k0 + kk, n = n0 + nn;
-      sB[nn][kk] = (k < ke && n < N) ? B[(long)k * sbk + (long)n * sbn] : (uint16_t)0;
+    if (kfull && n0 + 64 <= N && b_k) {  // a column's 8 consecutive k
+      const int nn = tid >> 2, kq = (tid & 3) * 8;
+      *reinterpret_cast<uint4*>(&sB[nn][kq]) =
+          *reinterpret_cast<const uint4*>(B + (long)(n0 + nn) * sbn + (k0 + kq));
+    } else if (kfull && n0 + 64 <= N && b_n) {  // 8 consecutive n at one k
+      const int kk = tid >> 3, nq = (tid & 7) * 8;
+      const uint4 v = *reinterpret_cast<const uint4*>(B + (long)(k0 + kk) * sbk + (n0 + nq));
+      uint16_t h[8];
+      __builtin_memcpy(h, &v, 16);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sB[nq + j][kk] = h[j];
+    } else {
+      for (int e = tid; e < 64 * 32; e += 256) {
+        const int nn = sbn == 1 ? e % 64 : e / 32, kk = sbn == 1 ? e / 64 : e % 32;
+        const int k = k0 + kk, n = n0 + nn;
+        sB[nn][kk] = (k < ke && n < N) ? B[(long)k * sbk + (long)n * sbn] : (uint16_t)0;
+      }
     }
     __syncthreads();
     const int r = lane & 15, kq = (lane >> 4) * 8;
