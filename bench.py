@@ -57,8 +57,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the config 3 (4K) and config 4 (12 streams) runs")
+    ap.add_argument("--config-steps", type=int, default=4, help="timed frames per config 3 / config 4 run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
-    ap.add_argument("--cpu-frames", type=int, default=2)
     ap.add_argument("--top-k", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--workload", default=WORKLOAD)
@@ -67,31 +68,93 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(spec, frames, n_frames):
-    """The reference encoder compiled from its sources (oracle/_ref), or the C
-    oracle where that was not built, on the first n_frames frames."""
+def _cpu_sample(args):
+    """One CPU worker: the reference encoder (oracle/_ref, compiled from its
+    sources), or the C oracle where that was not built, on two frames of the
+    workload starting at `first` -- an intra frame, then an inter frame
+    against it, as the reference's own frame-chunk threading restarts each
+    chunk with an intra frame (codec/codec.cpp:1775-1856).  Returns the
+    seconds of the intra and of the inter frame (fit + Reencode; PNG decode
+    excluded for the reference) and the kind."""
+    spec, first = args
     import oracle
-    nb = (spec["width"] // 4) * (spec["height"] // 4)
-    sample = frames[:n_frames]
+    from oracle import SynthCfg
+    O = oracle.Oracle()
+    cfg = SynthCfg(spec["width"], spec["height"], spec["seed"], spec["n_rects"], spec["vel_mode"], spec["vel"],
+                   8, spec["cut_permille"])
+    frames = np.stack([O.synth_frame(cfg, first + t) for t in range(2)])
     if oracle.reference_available():
-        R = oracle.Reference()
-        res = R.encode_sequence(sample, spec["S"], spec["thr"], spec["intra_interval"])
-        t = sum(r["t_fit"] + r["t_reencode"] for r in res)
-        kind = "reference"
-    else:
-        O = oracle.Oracle()
+        res = oracle.Reference().encode_sequence(frames, spec["S"], spec["thr"], 2)
+        return res[0]["t_fit"] + res[0]["t_reencode"], res[1]["t_fit"] + res[1]["t_reencode"], "reference"
+    ts = []
+    prev = None
+    for f in range(2):
         t0 = time.perf_counter()
-        O.encode_sequence(sample, spec["S"], spec["thr"], spec["intra_interval"])
-        t = time.perf_counter() - t0
-        kind = "port"
-    return {"value": n_frames * nb / t / 1e6, "unit": "Mblocks/s", "cores": 1, "kind": kind,
-            "sample": f"first {n_frames} frames of the same 1080p sequence (frame 0 intra, then inter), "
-                      f"{n_frames * nb} blocks, {t:.1f} s single-threaded; fit (PNG decode excluded) + Reencode"}
+        fit = O.fit_frame(frames[f])
+        r = O.reencode(frames[f], fit, spec["S"], spec["thr"], f == 0, prev)
+        ts.append(time.perf_counter() - t0)
+        prev = r["phys"]
+    return ts[0], ts[1], "port"
+
+
+def _cpu_info():
+    model, mhz = "unknown", None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            if line.startswith("cpu MHz") and mhz is None:
+                mhz = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return model, mhz
+
+
+def cpu_baseline(spec, steps):
+    """The reference CPU encoder on this host, on one core and on every core
+    this process may use (at most 16, the host share of one GPU), in worker
+    processes started before the GPU is touched.  Each worker times one
+    two-frame chunk (intra, inter) of the workload; a worker's rate is taken
+    at the intra/inter mix of the GPU's timed frames (the first `steps`
+    frames of the sequence, intra every intra_interval), and the all-core
+    rate is the sum over concurrently running workers."""
+    import multiprocessing as mp
+    nb = (spec["width"] // 4) * (spec["height"] // 4)
+    n_intra = sum(1 for f in range(steps) if (f % spec["frames"]) % spec["intra_interval"] == 0)
+    n_inter = steps - n_intra
+
+    def rate(t_i, t_p):
+        return steps * nb / (n_intra * t_i + n_inter * t_p) / 1e6
+
+    ctx = mp.get_context("fork")
+    with ctx.Pool(1) as pool:
+        t_i, t_p, kind = pool.map(_cpu_sample, [(spec, 0)])[0]
+    try:
+        n_cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n_cores = os.cpu_count() or 1
+    workers = max(1, min(16, n_cores))
+    t0 = time.perf_counter()
+    firsts = [(2 * w) % (spec["frames"] - 1) for w in range(workers)]
+    with ctx.Pool(workers) as pool:
+        res = pool.map(_cpu_sample, [(spec, f) for f in firsts])
+    wall = time.perf_counter() - t0
+    model, mhz = _cpu_info()
+    one = {"value": rate(t_i, t_p), "unit": "Mblocks/s", "cores": 1, "kind": kind,
+           "sample": f"frames 0 (intra) and 1 (inter) of the same 1080p sequence: {t_i:.2f} s + {t_p:.2f} s "
+                     f"single-threaded (fit, PNG decode excluded, + Reencode), rate taken at the timed frames' "
+                     f"mix of {n_intra} intra and {n_inter} inter frames"}
+    allc = {"value": sum(rate(a, b) for a, b, _ in res), "unit": "Mblocks/s", "cores": workers, "kind": kind,
+            "sample": f"{workers} worker processes at once, each a two-frame chunk (intra, inter) starting at "
+                      f"frames {firsts}; {wall:.1f} s wall; per-worker rates at the timed mix, summed"}
+    host = {"cpu_model": model, "cpu_mhz": mhz, "nproc": os.cpu_count(), "usable_cores": n_cores}
+    return one, allc, host
 
 
 KERNELS = ("fit", "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine", "intra_apply",
            "intra_reeval", "intra_select", "commit", "unique", "net_input", "net_l12", "net_l3",
-           "propose_inter", "top_intra", "lintra_pass", "pruned_inter", "pruned_intra", "fold_intra")
+           "propose_inter", "top_intra", "lintra_pass", "pruned_inter", "pruned_intra", "fold_intra",
+           "intra_fixpoint")
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "encode_1080p_plain_hbm_traffic.md")
@@ -185,6 +248,73 @@ class Runner:
         enc.profile_enable(False)
         return out, (t1 - t0) / steps * 1e3
 
+    def verify(self, steps, fixture):
+        """Compares the frames of the last run (steps frames from frame 0) with
+        the reference's per-frame SHA-256 of blocks, motion bytes and unique
+        palette; outside the timed region.  Returns (all equal, frames checked)."""
+        if fixture is None:
+            return None, 0
+        n = min(steps, self.n_seq, self.n_out, len(fixture["frames"]))
+        ok = True
+        for f in range(n):
+            o = f % self.n_out
+            nu = int(self.d_nuniq.download(np.uint32, 1, offset=4 * o)[0])
+            want = fixture["frames"][f]
+            ok = ok and _sha(self.d_phys[o].download(np.uint64, self.nb)) == want["phys_sha"]
+            ok = ok and _sha(self.d_motion[o].download(np.uint8, 2 * self.nb)) == want["motion_sha"]
+            ok = ok and _sha(self.d_unique[o].download(np.uint32, nu)) == want["unique_sha"]
+        return ok, n
+
+    def end_to_end(self, steps):
+        """The same steps fed from pinned host memory: per frame the RGB8
+        frame is copied to the device, encoded, and the blocks, motion bytes,
+        unique palette (full capacity) and its count are copied back, all on
+        the step's stream.  Returns seconds for `steps` frames."""
+        hip = self.hip
+        fb = self.W * self.H * 3
+        n_in = min(steps, self.n_seq)
+        h_in = hip.HostBuffer(n_in * fb)
+        h_in.array()[:] = self.frames[:n_in].reshape(-1)
+        h_out = hip.HostBuffer(self.nb * (8 + 2 + 4) + 4)
+        d_rgb = [hip.Buffer(fb) for _ in range(2)]
+        out_bytes = self.nb * 14 + 4
+
+        def step(f, set_intra):
+            fi = f % n_in
+            slot = f % 2
+            o = f % self.n_out
+            hip.memcpy_async(d_rgb[slot].ptr, h_in.ptr + fi * fb, fb, hip.hipMemcpyHostToDevice, self.stream)
+            self.enc.encode_frame_device(d_rgb[slot], self.W, self.H, set_intra,
+                                         None if set_intra else self.d_phys[(f - 1) % self.n_out], self.d_phys[o],
+                                         self.d_motion[o], self.d_unique[o], self.d_nuniq.offset(4 * o), None, None,
+                                         stream=self.stream.handle)
+            base = h_out.ptr
+            hip.memcpy_async(base, self.d_phys[o].ptr, 8 * self.nb, hip.hipMemcpyDeviceToHost, self.stream)
+            hip.memcpy_async(base + 8 * self.nb, self.d_motion[o].ptr, 2 * self.nb, hip.hipMemcpyDeviceToHost,
+                             self.stream)
+            hip.memcpy_async(base + 10 * self.nb, self.d_unique[o].ptr, 4 * self.nb, hip.hipMemcpyDeviceToHost,
+                             self.stream)
+            hip.memcpy_async(base + 14 * self.nb, self.d_nuniq.ptr + 4 * o, 4, hip.hipMemcpyDeviceToHost,
+                             self.stream)
+
+        def run(n):
+            set_intra, num = True, 1
+            for f in range(n):
+                if f % n_in == 0:
+                    set_intra, num = True, 1
+                step(f, set_intra)
+                set_intra, num = self.lib.next_intra(set_intra, num, self.spec["intra_interval"])
+
+        run(2)
+        self.stream.synchronize()
+        t0 = time.perf_counter()
+        run(steps)
+        self.stream.synchronize()
+        t = time.perf_counter() - t0
+        h_in.free()
+        h_out.free()
+        return t, fb + out_bytes
+
     def quality(self, steps):
         """Mean decoder-side PSNR of the frames the last run wrote (the last
         min(steps, n_out) of them), and their unique-block count."""
@@ -195,6 +325,19 @@ class Runner:
             phys = self.d_phys[f % self.n_out].download(np.uint64, self.nb)
             ps.append(O.psnr_physical(self.frames[f % self.n_seq], phys))
         return float(np.mean(ps))
+
+
+def _sha(a) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def load_fixture(workload):
+    p = os.path.join(ROOT, "tests", "golden", f"seq_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        return json.load(fh)
 
 
 def work_of(stats):
@@ -261,15 +404,148 @@ def train_networks(hip, lib, enc, spec, train_steps):
     return info
 
 
+def dominant_roofline(kernels, work):
+    """Roofline of the search kernel with the most time (work-done convention:
+    exact evaluations x W_CAND, bounds x W_BOUND)."""
+    ev, bd = W_CAND, W_BOUND
+    table = {
+        "inter": work["inter_candidates"] * ev,
+        "intra_dense": work["intra_candidates"] * ev,
+        "intra_reeval": work["reeval_candidates"] * ev,
+        "pruned_inter": work["inter_candidates"] * bd + work["exact_inter_candidates"] * ev,
+        "pruned_intra": work["intra_candidates"] * bd + work["exact_intra_candidates"] * ev,
+    }
+    names = [k for k in kernels if k in table]
+    if not names:
+        return None
+    dom = max(names, key=lambda k: kernels[k]["total_ms"])
+    ach = table[dom] / (kernels[dom]["total_ms"] * 1e-3) / 1e12
+    return {"kernel": dom, "bound": "valu", "achieved": round(ach, 3), "peak": VALU_PEAK, "unit": "T lane-op/s",
+            "frac": round(ach / VALU_PEAK, 4), "share_of_step": None}
+
+
+def measure_config3(hip, lib, steps, warmup):
+    """3840x2160, S = 32 (64x64-block window), thresholds 0, 8, 32, 128 (the
+    reference's floor(SSE/48) units): the dense search kernels.  Per threshold
+    the Mblocks/s of `steps` frames from frame 0, the dominant kernel's
+    roofline, and the first frames against the reference's fixture."""
+    from mptc_amd.workloads import SEQUENCES, synth_cfg
+    names = ("c3_t0", "c3_t8", "c3_t32", "c3_t128")
+    spec0 = SEQUENCES[names[0]]
+    W, H = spec0["width"], spec0["height"]
+    nb = (W // 4) * (H // 4)
+    frames = lib.synth_frames(synth_cfg(spec0), 0, max(steps, warmup, 2))
+    enc = lib.Encoder(W, H, spec0["S"], spec0["thr"])
+    R = Runner(hip, lib, enc, spec0, frames, max(steps, warmup, 2))
+    out = {}
+    for name in names:
+        spec = SEQUENCES[name]
+        enc.set_params(spec["S"], spec["thr"])
+        R.spec = spec
+        el = R.timed(steps, warmup)
+        ok, n = R.verify(steps, load_fixture(name))
+        k, _ = R.kernels(steps)
+        w = work_of(R.run(steps, collect=True))
+        roof = dominant_roofline(k, w)
+        if roof:
+            roof["share_of_step"] = round(k[roof["kernel"]]["total_ms"] / sum(v["total_ms"] for v in k.values()), 3)
+        out[name] = {"value": round(steps * nb / el / 1e6, 3), "ms_per_step": round(el / steps * 1e3, 3),
+                     "threshold": spec["thr"], "bit_exact": ok, "bit_exact_frames_checked": n, "roofline": roof,
+                     "kernels": {x: k[x] for x in k if k[x]["total_ms"] > 0.01 * steps},
+                     "candidates_per_frame": round((w["inter_candidates"] + w["intra_candidates"]) / steps),
+                     "max_intra_iterations": w["max_intra_iterations"]}
+    enc.close()
+    return {"workload": f"config 3: {W}x{H} RGB8, S={spec0['S']} (64x64-block window), seeded generator, "
+                        f"{steps} timed frames from frame 0 per threshold", "blocks_per_frame": nb, "results": out}
+
+
+def measure_config4(hip, lib, steps, warmup):
+    """Twelve streams of uneven size (3 each of 640x360, 1280x720, 1920x1080,
+    3840x2160; S = 8, threshold 8) encoded together: one step encodes the
+    next frame of every stream, every kernel of the step covering all twelve
+    frames.  The launch shape (graph replay included) is autotuned on the
+    batch first.  Reports Mblocks/s summed over the streams and checks the
+    first frames of every stream against the reference's fixtures."""
+    from mptc_amd.workloads import BATCH_STREAMS, SEQUENCES, synth_cfg
+    specs = [SEQUENCES[n] for n in BATCH_STREAMS]
+    nbs = [(s["width"] // 4) * (s["height"] // 4) for s in specs]
+    total = sum(nbs)
+    n_fr = max(steps, warmup, 3)
+    enc = lib.Encoder(3840, 2160, specs[0]["S"], specs[0]["thr"], max_batch_frames=len(specs),
+                      max_batch_blocks=total)
+    st = []
+    for s, nb in zip(specs, nbs):
+        fr = lib.synth_frames(synth_cfg(s), 0, n_fr)
+        st.append(dict(rgb=hip.Buffer.from_array(fr), w=s["width"], h=s["height"], nb=nb,
+                       phys=[hip.Buffer(8 * nb) for _ in range(n_fr)], motion=[hip.Buffer(2 * nb) for _ in range(n_fr)],
+                       unique=[hip.Buffer(4 * nb) for _ in range(n_fr)], nu=hip.Buffer(4 * n_fr)))
+    stream = hip.Stream()
+
+    def ios(f, set_intra):
+        return [dict(d_rgb=x["rgb"].offset(f * x["w"] * x["h"] * 3), width=x["w"], height=x["h"],
+                     is_intra=set_intra, d_ref=None if set_intra else x["phys"][f - 1], d_phys=x["phys"][f],
+                     d_motion=x["motion"][f], d_unique=x["unique"][f], d_n_unique=x["nu"].offset(4 * f))
+                for x in st]
+
+    def run(n):
+        set_intra, num = True, 1
+        for f in range(n):
+            enc.encode_batch_device(ios(f, set_intra), stream=stream.handle)
+            set_intra, num = lib.next_intra(set_intra, num, specs[0]["intra_interval"])
+
+    run(1)
+    stream.synchronize()
+    tuning, _ = enc.autotune(ios(1, False), steps=2)
+    run(warmup)
+    stream.synchronize()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    run(steps)
+    stream.synchronize()
+    hip.synchronize()
+    el = time.perf_counter() - t0
+    ok, checked = True, 0
+    for name, x in zip(BATCH_STREAMS, st):
+        fx = load_fixture(name)
+        for f in range(min(steps, len(fx["frames"]))):
+            nu = int(x["nu"].download(np.uint32, 1, offset=4 * f)[0])
+            want = fx["frames"][f]
+            ok = ok and _sha(x["phys"][f].download(np.uint64, x["nb"])) == want["phys_sha"]
+            ok = ok and _sha(x["motion"][f].download(np.uint8, 2 * x["nb"])) == want["motion_sha"]
+            ok = ok and _sha(x["unique"][f].download(np.uint32, nu)) == want["unique_sha"]
+            checked += 1
+    enc.profile_enable(True)
+    run(steps)
+    stream.synchronize()
+    k = {}
+    for name in KERNELS:
+        n, ms = enc.profile_read(name)
+        if n:
+            k[name] = {"launches": n, "total_ms": round(ms, 4), "avg_ms": round(ms / n, 5)}
+    enc.profile_enable(False)
+    stream.destroy()
+    enc.close()
+    return {"workload": "config 4: 12 streams (3 each of 640x360, 1280x720, 1920x1080, 3840x2160), S=8, "
+                        f"threshold 8, one step = the next frame of every stream, {steps} timed steps",
+            "blocks_per_step": total, "value": round(steps * total / el / 1e6, 3), "unit": "Mblocks/s",
+            "ms_per_step": round(el / steps * 1e3, 3), "tuning": tuning, "bit_exact": ok,
+            "bit_exact_frames_checked": checked, "kernels": k}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
-    from mptc_amd import hip, lib
     from mptc_amd.workloads import SEQUENCES, synth_cfg
 
+    # the CPU baseline first, in worker processes forked before any GPU work
+    cpu = None
+    if not a.no_cpu_baseline and not a.copy_calibration:
+        cpu = cpu_baseline(SEQUENCES[a.workload], a.steps)
+
+    from mptc_amd import hip, lib
     hip.set_device(0)
     if a.copy_calibration:
         n = 256 << 20
@@ -292,6 +568,10 @@ def main():
     tuning, tune_ms = R.autotune() if not a.no_autotune else (enc.get_tuning(), None)
     elapsed = R.timed(a.steps, a.warmup)
     value = a.steps * nb / elapsed / 1e6
+    # outside the timed region: the timed frames against the reference's fixture
+    bit_exact, n_checked = R.verify(a.steps, load_fixture(a.workload))
+    # the same frames fed from and returned to pinned host memory
+    e2e_s, e2e_bytes = R.end_to_end(a.steps)
     kernels, ms_ev = R.kernels(a.steps)
     work = work_of(R.run(a.steps, collect=True))
     psnr_plain = R.quality(a.steps)
@@ -317,7 +597,17 @@ def main():
             "unit": "T lane-op/s", "frac": round(achieved / VALU_PEAK, 4), "traffic": profiled_traffic(dom),
             "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
             "work_per_launch": " + ".join(f"{n / n_launch:.4g} candidate {what} x {w} lane-ops"
-                                          for n, what, w in parts)}
+                                          for n, what, w in parts),
+            "frac_is": "work done: every candidate the kernel bounds x W_BOUND + every exact evaluation x W_CAND"}
+    # the same launch time against two other work conventions: every window
+    # candidate at W_CAND (the full-window count, which pruning skips most of)
+    # and the exact evaluations alone
+    cand_key = {"inter": "inter_candidates", "pruned_inter": "inter_candidates", "intra_dense": "intra_candidates",
+                "pruned_intra": "intra_candidates", "intra_reeval": "reeval_candidates"}[dom]
+    exact_key = {"pruned_inter": "exact_inter_candidates", "pruned_intra": "exact_intra_candidates"}.get(dom, cand_key)
+    per_s = n_launch * kernels[dom]["avg_ms"] * 1e-3 * 1e12
+    roof["frac_full_window"] = round(work[cand_key] * ev / per_s / VALU_PEAK, 4)
+    roof["frac_exact_evaluations"] = round(work[exact_key] * ev / per_s / VALU_PEAK, 4)
 
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "Mblocks/s", "n_gpus": 1,
@@ -334,6 +624,12 @@ def main():
         "ms_per_step_with_kernel_events": round(ms_ev, 4),
         "work": work,
         "psnr_db": round(psnr_plain, 4),
+        "bit_exact": bit_exact, "bit_exact_frames_checked": n_checked,
+        "end_to_end": {"value": round(a.steps * nb / e2e_s / 1e6, 4), "unit": "Mblocks/s",
+                       "ms_per_step": round(e2e_s / a.steps * 1e3, 4),
+                       "note": f"same steps with the RGB8 frame copied in from pinned host memory and blocks, "
+                               f"motion, unique palette and count copied back per step "
+                               f"({e2e_bytes} bytes over PCIe per frame)"},
     }
 
     if not a.no_learned:
@@ -386,9 +682,17 @@ def main():
         out["learned"] = learned
         enc.set_search_mode(lib.SEARCH_EXHAUSTIVE, 8)
 
-    if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(spec, frames, a.cpu_frames)
-        out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    if not a.no_configs:
+        out["config3"] = measure_config3(hip, lib, steps=a.config_steps, warmup=1)
+        out["config4"] = measure_config4(hip, lib, steps=a.config_steps, warmup=1)
+
+    if cpu is not None:
+        one, allc, host = cpu
+        out["cpu_baseline"] = allc
+        out["cpu_baseline_1core"] = one
+        out["cpu_host"] = host
+        out["gpu_over_cpu"] = round(value / allc["value"], 1)
+        out["gpu_over_cpu_1core"] = round(value / one["value"], 1)
     print(json.dumps(out), flush=True)
     enc.close()
 

@@ -61,6 +61,13 @@ typedef struct {
   int32_t prune;      /* 1: for S <= 8 and thresholds 0..4, evaluate exactly only the
                          candidates an error lower bound cannot rule out (same output);
                          0: evaluate every candidate */
+  int32_t fixpoint;   /* intra fixpoint of the exhaustive search: 1 = one persistent
+                         (cooperative) launch runs every pass and the call returns as
+                         soon as the step is queued; 2 = the same, each pass also
+                         handing every chain of reusing blocks its root's word at once
+                         (for long reuse chains: large windows, high thresholds);
+                         0 = passes queued in batches of pass_batch, the host checking
+                         convergence between batches */
 } mptc_tuning;
 
 /* One frame of a batched step: device pointers, layouts as mptc_reencode. */
@@ -126,7 +133,13 @@ int32_t mptc_reencode(mptc_ctx* ctx, const uint8_t* rgb, int32_t width, int32_t 
  * d_eps_out and d_kind_out may be NULL.  Work is queued on `hip_stream`
  * (a hipStream_t; NULL = the context's stream); the call returns once the
  * frame's work is queued and its intra fixpoint is reached (it synchronises
- * with the stream to test convergence).  Same layouts as mptc_reencode. */
+ * with the stream to test convergence; with the persistent fixpoint, the
+ * default, it returns as soon as the step is queued).  Errors of queued work
+ * (a fixpoint that did not settle) are reported by the context's next call,
+ * mptc_synchronize or mptc_get_stats.  Same layouts as mptc_reencode.
+ * Successive calls on one context may use different streams: a call on
+ * another stream than the previous call's first waits for that call's work
+ * (the context's buffers are shared), so outputs never race. */
 int32_t mptc_encode_frame_device(mptc_ctx* ctx, const uint8_t* d_rgb, int32_t width,
                                  int32_t height, int32_t is_intra, const uint64_t* d_ref_phys,
                                  uint64_t* d_phys_out, uint8_t* d_motion_out,
@@ -142,7 +155,8 @@ int32_t mptc_encode_batch_device(mptc_ctx* ctx, int32_t n_frames, const mptc_fra
                                  void* hip_stream);
 
 /* Launch-shape tuning.  mptc_autotune times every candidate shape (wg_size x
- * tile_w x win_chunks x use_graph, and prune 0/1 where the pruned search applies) on
+ * tile_w x win_chunks x use_graph, prune 0/1 where the pruned search applies and
+ * fixpoint 0/1/2 in the exhaustive search) on
  * `steps` encodes of the given batch (outputs are overwritten), keeps the
  * fastest and reports it. */
 int32_t mptc_get_tuning(mptc_ctx* ctx, mptc_tuning* out);
@@ -151,7 +165,8 @@ int32_t mptc_autotune(mptc_ctx* ctx, int32_t n_frames, const mptc_frame_io* fram
                       mptc_tuning* best, double* best_ms_per_step);
 
 /* Waits for all work queued on the context's own stream (the stream used when
- * a device entry point is given hip_stream = NULL). */
+ * a device entry point is given hip_stream = NULL) and for the last call's
+ * work on whatever stream it used. */
 int32_t mptc_synchronize(mptc_ctx* ctx);
 
 /* Counters of the last step encoded with this context, summed over its
@@ -166,7 +181,8 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
  * "intra_apply", "intra_reeval", "intra_select", "commit", "unique";
  * learned search: "net_input", "net_l12", "net_l3" (inference), "propose_inter",
  * "top_intra", "lintra_pass"; "net_forward": training products; pruned search:
- * "pruned_inter", "pruned_intra", "fold_intra") since it was enabled
+ * "pruned_inter", "pruned_intra", "fold_intra"; the persistent fixpoint:
+ * "intra_fixpoint") since it was enabled
  * (synchronises).  While enabled, steps run as individual launches, not graphs. */
 int32_t mptc_profile_enable(mptc_ctx* ctx, int32_t enable);
 int32_t mptc_profile_read(mptc_ctx* ctx, const char* name, int64_t* launches, double* total_ms);

@@ -79,12 +79,14 @@ uint8_t snap_bits(uint8_t x, uint8_t keep, uint8_t half, int shift) {
   return high;
 }
 
+constexpr int kMaxDevices = 64;  // HIP device ordinals this library serves
+
 int init_tables(int device) {
   static std::mutex mu;
-  static std::vector<int> done;
+  static bool done[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return fail(MPTC_EINVAL, "device ordinal above 63");
   std::lock_guard<std::mutex> lk(mu);
-  for (int d : done)
-    if (d == device) return MPTC_OK;
+  if (done[device]) return MPTC_OK;
   uint8_t e5[32], e6[64], om5[256][2], om6[256][2], snap[512];
   for (int i = 0; i < 32; ++i) e5[i] = (uint8_t)((i << 3) | (i >> 2));
   for (int i = 0; i < 64; ++i) e6[i] = (uint8_t)((i << 2) | (i >> 4));
@@ -97,21 +99,21 @@ int init_tables(int device) {
   HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_omatch5), om5, sizeof(om5)));
   HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_omatch6), om6, sizeof(om6)));
   HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_snap), snap, sizeof(snap)));
-  done.push_back(device);
+  done[device] = true;
   return MPTC_OK;
 }
 
 enum KernelId {
   KF_FIT, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
   KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
-  KF_NET_L3, KF_PRUNED_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
+  KF_NET_L3, KF_PRUNED_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_FIXPOINT, KF_N
 };
 const char* kKernelNames[KF_N] = {"fit",          "inter",
                                   "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
                                   "intra_apply",  "intra_reeval", "intra_select", "commit",
                                   "unique",       "net_input",   "net_forward", "propose_inter",
                                   "top_intra",    "lintra_pass",  "net_l12",     "net_l3",
-                                  "pruned_inter", "pruned_intra", "fold_intra"};
+                                  "pruned_inter", "pruned_intra", "fold_intra", "intra_fixpoint"};
 constexpr int kMaxChunks = 8;  // window-row chunks per block in the search kernels
 
 constexpr int kReevalGrid = 4096;  // workgroups of the re-evaluation pass (grid-stride)
@@ -173,7 +175,12 @@ struct mptc_ctx {
   int max_frames = 1;
   size_t nbs = 0;     // E row stride (blocks)
   hipStream_t stream = nullptr;
-  mptc_tuning tune{256, 16, 2, 0, 4, 1};  // the fastest 1080p shape measured on MI355X
+  // Every call queues its work on one stream (the caller's or `stream`) and
+  // records done_ev there; a call on another stream first waits for it, so
+  // work on the context's buffers is ordered whatever streams callers use.
+  hipEvent_t done_ev = nullptr;
+  hipStream_t done_stream = nullptr;  // the stream done_ev was last recorded on (null: never)
+  mptc_tuning tune{256, 16, 2, 0, 4, 1, 1};  // the fastest 1080p shape measured on MI355X
   // device buffers (one entry per block of the batch unless noted)
   FrameDev* d_fr = nullptr;  // kMaxFrames descriptors
   FrameDev* h_fr = nullptr;  // pinned staging ring: 4 x kMaxFrames
@@ -183,6 +190,7 @@ struct mptc_ctx {
   uint32_t *d_rw = nullptr, *d_cw = nullptr, *d_cwn = nullptr, *d_dirty = nullptr, *d_tiles = nullptr;
   float4 *d_rtab = nullptr, *d_ctab = nullptr;
   int *d_orig = nullptr, *d_chg = nullptr;
+  int* d_anc = nullptr;  // chain jumping of the persistent fixpoint: 2 x nbs pointers
   uint2* d_E = nullptr;  // S^2 groups of four int16 error deltas per block
   uint4* d_summ = nullptr;  // kMaxChunks selection summaries per block
   DevStats* d_stats = nullptr;
@@ -197,7 +205,8 @@ struct mptc_ctx {
   uint32_t *d_unique = nullptr, *d_nunique = nullptr;
   // batch shape the maps and graphs were built for
   std::vector<int> shape;
-  mptc_tuning shape_tune{0, 0, 0, 0, 0, 0};
+  mptc_tuning shape_tune{0, 0, 0, 0, 0, 0, 0};
+  int fix_grid = 0;  // workgroups of the persistent fixpoint launch (all resident)
   int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0, n_ltiles = 0;
   GraphSet graphs;
   bool any_inter = true;  // the batch holds an inter frame
@@ -267,6 +276,20 @@ int collect_profile(mptc_ctx* c) {
   return MPTC_OK;
 }
 
+// Orders work about to be queued on `s` after the context's previous call,
+// which may have used another stream.
+int join_stream(mptc_ctx* c, hipStream_t s) {
+  if (c->done_stream && c->done_stream != s) HIP_TRY(hipStreamWaitEvent(s, c->done_ev, 0));
+  return MPTC_OK;
+}
+
+// Marks the end of a call's work on `s` (see join_stream).
+int leave_stream(mptc_ctx* c, hipStream_t s) {
+  HIP_TRY(hipEventRecord(c->done_ev, s));
+  c->done_stream = s;
+  return MPTC_OK;
+}
+
 int check_params(int S, int thr) {
   if (S < 1 || S > kMaxS) return fail(MPTC_EINVAL, "search_area must be in 1..63");
   if (thr < -32768 || thr > 30000) return fail(MPTC_EINVAL, "err_threshold must be in -32768..30000");
@@ -281,6 +304,7 @@ int check_tuning(const mptc_tuning& t) {
   if (t.pass_batch < 1 || t.pass_batch > 16) return fail(MPTC_EINVAL, "pass_batch must be in 1..16");
   if (t.win_chunks < 1 || t.win_chunks > kMaxChunks) return fail(MPTC_EINVAL, "win_chunks must be in 1..8");
   if (t.prune != 0 && t.prune != 1) return fail(MPTC_EINVAL, "prune must be 0 or 1");
+  if (t.fixpoint < 0 || t.fixpoint > 2) return fail(MPTC_EINVAL, "fixpoint must be 0, 1 or 2");
   return MPTC_OK;
 }
 
@@ -491,11 +515,33 @@ struct FrameIn {
   uint8_t* kind;
 };
 
+// The device descriptor of one frame of a batch whose earlier frames hold
+// `base` blocks, `tb` unique tiles, `wt` wave tiles of width tw and `lt`
+// learned-pass tiles; advances the four counts past this frame.
+FrameDev frame_dev(const FrameIn& x, int tw, int& base, int& tb, int& wt, int& lt) {
+  const int th = 64 / tw;
+  FrameDev d{};
+  d.rgb = x.rgb; d.ref = x.ref; d.phys_out = x.phys; d.motion = x.motion; d.unique = x.unique;
+  d.n_unique = x.n_unique; d.eps = x.eps; d.kind = x.kind;
+  d.width = x.w; d.bw = x.w / 4; d.bh = x.h / 4; d.nb = d.bw * d.bh;
+  d.base = base; d.is_intra = x.is_intra ? 1 : 0;
+  d.tile_base = tb; d.n_tiles = (d.nb + kUniqueTile - 1) / kUniqueTile;
+  d.tiles_x = (d.bw + tw - 1) / tw;
+  d.wtile_base = wt;
+  d.ltile_base = lt;
+  d.ltiles_x = (d.bw + 7) / 8;
+  lt += d.ltiles_x * ((d.bh + 7) / 8);
+  base += d.nb;
+  tb += d.n_tiles;
+  wt += d.tiles_x * ((d.bh + th - 1) / th);
+  return d;
+}
+
 // Lays out a batch (block, wave-tile and unique-tile offsets) and, when its
 // shape changed, rebuilds the block/tile frame maps and drops the graphs.
 int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
   if (n < 1 || n > c->max_frames) return fail(MPTC_EINVAL, "batch size out of range");
-  const int tw = c->tune.tile_w, th = 64 / tw;
+  const int tw = c->tune.tile_w;
   std::vector<int> shape;
   FrameDev* h = c->h_fr + (size_t)c->fr_slot * kMaxFrames;
   int base = 0, tb = 0, wt = 0, lt = 0;
@@ -505,20 +551,7 @@ int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
       return fail(MPTC_EINVAL, "width and height must be positive multiples of 4");
     if (!x.rgb || !x.phys || !x.motion || !x.unique || !x.n_unique) return fail(MPTC_EINVAL, "null argument");
     if (!x.is_intra && !x.ref) return fail(MPTC_EINVAL, "inter frame needs the previous frame's blocks");
-    FrameDev d{};
-    d.rgb = x.rgb; d.ref = x.ref; d.phys_out = x.phys; d.motion = x.motion; d.unique = x.unique;
-    d.n_unique = x.n_unique; d.eps = x.eps; d.kind = x.kind;
-    d.width = x.w; d.bw = x.w / 4; d.bh = x.h / 4; d.nb = d.bw * d.bh;
-    d.base = base; d.is_intra = x.is_intra ? 1 : 0;
-    d.tile_base = tb; d.n_tiles = (d.nb + kUniqueTile - 1) / kUniqueTile;
-    d.tiles_x = (d.bw + tw - 1) / tw;
-    d.wtile_base = wt;
-    d.ltile_base = lt;
-    d.ltiles_x = (d.bw + 7) / 8;
-    lt += d.ltiles_x * ((d.bh + 7) / 8);
-    base += d.nb;
-    tb += d.n_tiles;
-    wt += d.tiles_x * ((d.bh + th - 1) / th);
+    const FrameDev d = frame_dev(x, tw, base, tb, wt, lt);
     if (base > c->max_total) return fail(MPTC_EINVAL, "batch holds more blocks than the context allows");
     h[f] = d;
     shape.push_back(x.w);
@@ -810,11 +843,92 @@ int capture(hipStream_t s, hipGraphExec_t* exec, Q&& q) {
   return MPTC_OK;
 }
 
+// Sticky error of queued work: the persistent fixpoint kernel sets h_conv[1]
+// when a grid barrier timed out or the passes did not converge.
+int check_async_error(const mptc_ctx* c) {
+  const uint32_t e = *reinterpret_cast<volatile const uint32_t*>(c->h_conv + 1);
+  if (e & 1u) return fail(MPTC_EHIP, "intra fixpoint kernel: a grid barrier timed out");
+  if (e & 2u) return fail(MPTC_ESTATE, "intra fixpoint did not converge");
+  return MPTC_OK;
+}
+
+// Workgroups per compute unit of the persistent fixpoint launch (capped by
+// its occupancy): more hide the re-evaluations' latency, fewer make the grid
+// barriers cheaper.
+#ifndef MPTC_FIX_GROUPS_PER_CU
+#define MPTC_FIX_GROUPS_PER_CU 4
+#endif
+constexpr int kFixGroupsPerCu = MPTC_FIX_GROUPS_PER_CU;
+
+// All fixpoint passes in one cooperative launch (k_intra_fixpoint): every
+// workgroup resident, grid barriers between the phases.
+int queue_fixpoint(mptc_ctx* c, hipStream_t s) {
+  if (c->fix_grid == 0) {
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_intra_fixpoint),
+                                                         256, 0));
+    if (per_cu < 1) return fail(MPTC_EHIP, "persistent fixpoint kernel does not fit a compute unit");
+    c->fix_grid = c->n_cu * std::min(per_cu, kFixGroupsPerCu);
+  }
+  const FrameDev* fr = c->d_fr;
+  const uint8_t* sid = c->d_sid;
+  int total = c->total, S = c->cfg.search_area, thr = c->cfg.err_threshold;
+  const uint64_t* own = c->d_own;
+  const uint8_t* need = c->d_need;
+  uint32_t* cw = c->d_cw;
+  float4* ctab = c->d_ctab;
+  int* chg = c->d_chg;
+  uint2* E4 = c->d_E;
+  size_t nbs = c->nbs;
+  uint32_t* dirty = c->d_dirty;
+  const int* orig = c->d_orig;
+  uint64_t* dec = c->d_intra;
+  DevStats* st = c->d_stats;
+  uint32_t* pm = use_prune(c) ? c->d_pm : nullptr;
+  int max_passes = total + 8;  // the raster-triangular system settles within one pass per block
+  uint32_t* err_out = c->d_conv + 1;
+  int jump = c->tune.fixpoint == 2 ? 1 : 0;
+  int* anc = c->d_anc;
+  void* args[] = {&fr, &sid, &total, &S, &thr, &own, &need, &cw, &ctab, &chg, &E4, &nbs, &dirty, &orig, &dec, &st,
+                  &pm, &max_passes, &err_out, &jump, &anc};
+  hipError_t ce = hipSuccess;
+  int rc = launch(c, s, KF_FIXPOINT, [&] {
+    ce = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_intra_fixpoint), dim3(c->fix_grid), dim3(256),
+                                    args, 0, s);
+  });
+  if (ce != hipSuccess) return fail(MPTC_EHIP, std::string("hipLaunchCooperativeKernel: ") + hipGetErrorString(ce));
+  return rc;
+}
+
 // One encode step over the prepared batch.
 int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
   const bool graphs = c->tune.use_graph && !c->prof && fit;
   Graphs& gr = c->graphs.v[c->any_inter ? 1 : 0];
   int rc;
+  if ((rc = check_async_error(c))) return rc;
+  if (c->tune.fixpoint && c->mode == MPTC_SEARCH_EXHAUSTIVE) {
+    // prefix, every fixpoint pass in one persistent launch, suffix: no host
+    // wait; the call returns once the step is queued
+    if (graphs && !gr.prefix) {
+      if ((rc = capture(s, &gr.prefix, [&] { return queue_prefix(c, s, true); }))) return rc;
+      if ((rc = capture(s, &gr.suffix, [&] { return queue_suffix(c, s); }))) return rc;
+    }
+    if (graphs) {
+      HIP_TRY(hipGraphLaunch(gr.prefix, s));
+    } else if ((rc = queue_prefix(c, s, fit))) {
+      return rc;
+    }
+    if ((rc = queue_fixpoint(c, s))) return rc;
+    if (graphs) {
+      HIP_TRY(hipGraphLaunch(gr.suffix, s));
+    } else if ((rc = queue_suffix(c, s))) {
+      return rc;
+    }
+    c->last_passes = -1;
+    c->last_total = c->total;
+    c->last_recorded = c->record;
+    return MPTC_OK;
+  }
   // Passes that change words write their count to h_conv; a step whose first
   // search changed nothing has no such pass (its passes return at once).  No
   // pass of an earlier step can still write it: they ended at a zero count,
@@ -877,11 +991,15 @@ int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
 }
 
 int encode_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in, bool fit) {
-  int rc = prepare_batch(c, s, n, in);
+  // the previous step's suffix (commit, unique palettes) may still read the
+  // work buffers on another stream
+  int rc = join_stream(c, s);
   if (rc) return rc;
+  if ((rc = prepare_batch(c, s, n, in))) return rc;
   if (use_prune(c) && (rc = prune_ensure(c))) return rc;
   c->last_intra = n == 1 ? in[0].is_intra : 0;
-  return run_step(c, s, fit);
+  if ((rc = run_step(c, s, fit))) return rc;
+  return leave_stream(c, s);
 }
 
 int one_frame_dims(mptc_ctx* c, int w, int h) {
@@ -933,6 +1051,8 @@ int32_t mptc_create(const mptc_config* cfg, mptc_ctx** out) {
   };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(MPTC_EHIP, "hipStreamCreate failed"));
+  if (hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming) != hipSuccess)
+    return cleanup(fail(MPTC_EHIP, "hipEventCreate failed"));
   const int max_tiles = nb / kUniqueTile + frames + 1;
   const size_t groups = (size_t)cfg->search_area * cfg->search_area;
   const size_t one_nb = (size_t)one;
@@ -941,6 +1061,7 @@ int32_t mptc_create(const mptc_config* cfg, mptc_ctx** out) {
       (rc = dalloc(&c->d_rw, nb)) || (rc = dalloc(&c->d_cw, nb)) || (rc = dalloc(&c->d_cwn, nb)) ||
       (rc = dalloc(&c->d_dirty, nb)) || (rc = dalloc(&c->d_tiles, max_tiles)) || (rc = dalloc(&c->d_rtab, nb)) ||
       (rc = dalloc(&c->d_ctab, nb)) || (rc = dalloc(&c->d_orig, nb)) || (rc = dalloc(&c->d_chg, nb)) ||
+      (rc = dalloc(&c->d_anc, 2 * c->nbs)) ||
       (rc = dalloc(&c->d_E, groups * c->nbs)) || (rc = dalloc(&c->d_summ, kMaxChunks * c->nbs)) || (rc = dalloc(&c->d_stats, 1)) ||
       (rc = dalloc(&c->d_rgb, one_nb * 48)) || (rc = dalloc(&c->d_ref, one_nb)) || (rc = dalloc(&c->d_phys, one_nb)) ||
       (rc = dalloc(&c->d_motion, one_nb * 2)) || (rc = dalloc(&c->d_eps, one_nb * 6)) ||
@@ -954,6 +1075,7 @@ int32_t mptc_create(const mptc_config* cfg, mptc_ctx** out) {
           hipSuccess)
     return cleanup(fail(MPTC_ENOMEM, "hipHostMalloc failed"));
   std::memset(c->h_stats, 0, sizeof(DevStats));
+  std::memset(c->h_conv, 0, 64 * sizeof(uint32_t));
   *out = c;
   return MPTC_OK;
 }
@@ -962,13 +1084,15 @@ void mptc_destroy(mptc_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->cfg.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->done_stream) (void)hipEventSynchronize(c->done_ev);  // the last call may have used a caller's stream
+  if (c->done_ev) (void)hipEventDestroy(c->done_ev);
   c->graphs.destroy();
   for (auto& p : c->pending) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
   for (auto& p : c->spare) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
   for (hipEvent_t e : c->batch_ev) (void)hipEventDestroy(e);
   void* bufs[] = {c->d_fr,   c->d_sid,   c->d_tsid,   c->d_need,  c->d_own,   c->d_inter,  c->d_intra,
                   c->d_rw,   c->d_cw,    c->d_cwn,    c->d_dirty, c->d_tiles, c->d_rtab,   c->d_ctab,
-                  c->d_orig, c->d_chg,   c->d_E, c->d_summ,     c->d_stats, c->d_rgb,    c->d_ref,
+                  c->d_orig, c->d_chg,   c->d_E, c->d_summ,     c->d_stats, c->d_rgb,    c->d_ref, c->d_anc,
                   c->d_phys, c->d_motion, c->d_eps,   c->d_kind,  c->d_unique, c->d_nunique};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
@@ -1019,6 +1143,7 @@ int32_t mptc_fit(mptc_ctx* c, const uint8_t* rgb, int32_t w, int32_t h, uint64_t
   HIP_TRY(hipSetDevice(c->cfg.device));
   hipStream_t s = c->stream;
   const int nb = (w / 4) * (h / 4);
+  if ((rc = join_stream(c, s))) return rc;
   HIP_TRY(hipMemcpyAsync(c->d_rgb, rgb, (size_t)w * h * 3, hipMemcpyHostToDevice, s));
   FrameIn in{c->d_rgb, w, h, 1, nullptr, c->d_phys, c->d_motion, c->d_unique, c->d_nunique, nullptr, nullptr};
   if ((rc = prepare_batch(c, s, 1, &in))) return rc;
@@ -1027,6 +1152,7 @@ int32_t mptc_fit(mptc_ctx* c, const uint8_t* rgb, int32_t w, int32_t h, uint64_t
        })))
     return rc;
   HIP_TRY(hipMemcpyAsync(phys_out, c->d_own, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
+  if ((rc = leave_stream(c, s))) return rc;
   HIP_TRY(hipStreamSynchronize(s));
   return MPTC_OK;
 }
@@ -1042,6 +1168,7 @@ int32_t mptc_reencode(mptc_ctx* c, const uint8_t* rgb, int32_t w, int32_t h, int
   HIP_TRY(hipSetDevice(c->cfg.device));
   const int nb = (w / 4) * (h / 4);
   hipStream_t s = c->stream;
+  if ((rc = join_stream(c, s))) return rc;
   HIP_TRY(hipMemcpyAsync(c->d_rgb, rgb, (size_t)w * h * 3, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->d_own, phys_inout, (size_t)nb * 8, hipMemcpyHostToDevice, s));
   if (!is_intra) HIP_TRY(hipMemcpyAsync(c->d_ref, ref_phys, (size_t)nb * 8, hipMemcpyHostToDevice, s));
@@ -1101,12 +1228,15 @@ int32_t mptc_autotune(mptc_ctx* c, int32_t n, const mptc_frame_io* frames, int32
   probe.prune = 1;
   c->tune = probe;
   const int n_prune = use_prune(c) ? 2 : 1;
+  const int n_fix = c->mode == MPTC_SEARCH_EXHAUSTIVE ? 3 : 1;  // the persistent fixpoint applies there
+  for (int fx = 0; fx < n_fix; ++fx)
   for (int pr = 0; pr < n_prune; ++pr)
   for (int graph = 0; graph < 2; ++graph)
   for (int wg : wgs)
     for (int tw : tws)
       for (int kc : chunks) {
-        c->tune = mptc_tuning{wg, tw, keep.pass_batch, graph, kc, n_prune == 2 ? pr : keep.prune};
+        c->tune = mptc_tuning{wg, tw, keep.pass_batch, graph, kc, n_prune == 2 ? pr : keep.prune,
+                              n_fix == 3 ? (fx + 1) % 3 : keep.fixpoint};
         int rc = mptc_encode_batch_device(c, n, frames, nullptr);  // warm-up: maps and graphs
         if (rc) {
           c->tune = keep;
@@ -1137,13 +1267,16 @@ int32_t mptc_synchronize(mptc_ctx* c) {
   if (!c) return fail(MPTC_EINVAL, "null context");
   HIP_TRY(hipSetDevice(c->cfg.device));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  return MPTC_OK;
+  if (c->done_stream) HIP_TRY(hipEventSynchronize(c->done_ev));  // the last call's stream
+  return check_async_error(c);
 }
 
 int32_t mptc_get_stats(mptc_ctx* c, mptc_frame_stats* out) {
   if (!c || !out) return fail(MPTC_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(c->cfg.device));
   HIP_TRY(hipDeviceSynchronize());
+  int rc = check_async_error(c);
+  if (rc) return rc;
   DevStats d;
   HIP_TRY(hipMemcpy(&d, c->d_stats, sizeof(d), hipMemcpyDeviceToHost));
   out->n_blocks = c->last_total;
@@ -1198,6 +1331,10 @@ static int check_net(mptc_ctx* c, int net) {
   if (!c) return fail(MPTC_EINVAL, "null context");
   if (net != MPTC_NET_INTER && net != MPTC_NET_INTRA) return fail(MPTC_EINVAL, "net must be 0 (inter) or 1 (intra)");
   HIP_TRY(hipSetDevice(c->cfg.device));
+  // The network entry points run on the context's stream and synchronise it
+  // before they return; a step queued on a caller's stream may still be
+  // reading the weights or writing the outputs they read.
+  if (c->done_stream && c->done_stream != c->stream) HIP_TRY(hipEventSynchronize(c->done_ev));
   return net_alloc(c->net[net]);
 }
 
@@ -1381,8 +1518,7 @@ int32_t mptc_collect_pairs(mptc_ctx* c, int32_t net, uint8_t* d_px, uint32_t* d_
   if (rc) return rc;
   if (!d_px || !d_target || !d_eps_out || !d_count || capacity < 0) return fail(MPTC_EINVAL, "bad argument");
   if (!c->last_recorded) return fail(MPTC_ESTATE, "the last step did not record acceptance sets");
-  hipStream_t s = c->stream;
-  HIP_TRY(hipStreamSynchronize(s));  // the step may have run on a caller's stream
+  hipStream_t s = c->stream;  // check_net waited for a step queued on a caller's stream
   hipLaunchKernelGGL(k_collect_masks, dim3(grid_for(c->total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, c->total,
                      net, c->cfg.search_area, c->d_need, c->d_amask[net], c->d_own, d_px, d_target, d_eps_out,
                      d_count, capacity);

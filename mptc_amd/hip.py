@@ -113,6 +113,43 @@ class View:
         return self.buf.ptr + self.off
 
 
+class HostBuffer:
+    """Pinned (page-locked) host memory of `nbytes` bytes (hipHostMalloc), so
+    that copies to and from the device run asynchronously at full PCIe rate."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(rt().hipHostMalloc(C.byref(p), C.c_size_t(max(self.nbytes, 1)), C.c_uint(0)), "hipHostMalloc")
+        self.ptr = p.value
+
+    def array(self, dtype=np.uint8) -> np.ndarray:
+        """A numpy view of the whole buffer."""
+        n = self.nbytes // np.dtype(dtype).itemsize
+        return np.ctypeslib.as_array(C.cast(C.c_void_p(self.ptr), C.POINTER(C.c_uint8)),
+                                     shape=(self.nbytes,)).view(dtype)[:n]
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def free(self):
+        if self.ptr:
+            rt().hipHostFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def memcpy_async(dst: int, src: int, nbytes: int, kind: int, stream: "Stream"):
+    """hipMemcpyAsync between raw addresses (kind: hipMemcpyHostToDevice, ...)."""
+    check(rt().hipMemcpyAsync(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), C.c_int(kind),
+                              C.c_void_p(stream.handle)), "hipMemcpyAsync")
+
+
 class Stream:
     def __init__(self):
         s = C.c_void_p()

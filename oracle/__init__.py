@@ -223,7 +223,7 @@ class Reference:
         L.ref_eval_candidate.restype = C.c_int
         L.ref_encode_sequence.argtypes = [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                           C.c_int, C.c_char_p, _u64p, _u8p, _u32p, _i32p, _u8p,
-                                          _f64p, _f64p]
+                                          _f64p, _f64p, _f64p, _f64p]
         L.ref_encode_sequence.restype = C.c_int
         L.ref_time_fit.argtypes = [_u8p, C.c_int, C.c_int, _u64p]
         L.ref_time_fit.restype = C.c_double
@@ -242,7 +242,10 @@ class Reference:
         return (bool(ok), e.value, r.value, eps) if ok else (False, 0, 0, None)
 
     def encode_sequence(self, frames: np.ndarray, S: int, thr: int, intra_interval: int,
-                        tmpdir: str = "/tmp"):
+                        tmpdir: str = "/tmp", psnr: bool = False):
+        """psnr=True adds the reference's DXTImage::PSNR per frame: encoder
+        side (logical blocks after Reencode, `psnr_enc`) and decoder side
+        (after SetLogicalBlocks, `psnr_dec`)."""
         frames = np.ascontiguousarray(frames, dtype=np.uint8)
         n, h, w, _ = frames.shape
         nb = ((h + 3) // 4) * ((w + 3) // 4)
@@ -253,12 +256,16 @@ class Reference:
         eps = np.zeros(n * nb * 6, np.uint8)
         t_fit = np.zeros(n, np.float64)
         t_re = np.zeros(n, np.float64)
+        p_enc = np.zeros(n, np.float64)
+        p_dec = np.zeros(n, np.float64)
         d = os.path.join(tmpdir, f"mptc_ref_{os.getpid()}")
         os.makedirs(d, exist_ok=True)
         rc = self.L.ref_encode_sequence(_p(frames, _u8p), w, h, n, S, thr, intra_interval,
                                         d.encode(), _p(phys, _u64p), _p(motion, _u8p),
                                         _p(unique, _u32p), _p(n_unique, _i32p), _p(eps, _u8p),
-                                        _p(t_fit, _f64p), _p(t_re, _f64p))
+                                        _p(t_fit, _f64p), _p(t_re, _f64p),
+                                        _p(p_enc, _f64p) if psnr else None,
+                                        _p(p_dec, _f64p) if psnr else None)
         try:
             os.rmdir(d)
         except OSError:
@@ -272,6 +279,9 @@ class Reference:
                             unique=unique[f * nb:f * nb + n_unique[f]].copy(),
                             eps=eps[f * 6 * nb:(f + 1) * 6 * nb].copy(),
                             t_fit=float(t_fit[f]), t_reencode=float(t_re[f])))
+            if psnr:
+                out[-1]["psnr_enc"] = float(p_enc[f])
+                out[-1]["psnr_dec"] = float(p_dec[f])
         return out
 
     def time_fit(self, rgb):

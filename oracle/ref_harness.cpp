@@ -95,12 +95,18 @@ int ref_eval_candidate(uint64_t own_phys, const uint8_t px[48], uint32_t cand, i
 // re-encoded against the previous frame (codec/dxt_image.cpp:868).
 // Outputs per frame f (nb = blocks per frame):
 //   phys[f*nb..], motion[f*2nb..], unique[f*nb..] (n_unique[f] valid),
-//   eps[f*6nb..] (logical ep1/ep2 rgb after re-encode), t_fit[f], t_reencode[f].
+//   eps[f*6nb..] (logical ep1/ep2 rgb after re-encode), t_fit[f], t_reencode[f],
+//   psnr_enc[f]: DXTImage::PSNR (codec/dxt_image.cpp:363-383) right after
+//     Reencode, over the encoder's logical blocks (re-fit endpoints as the
+//     encoder keeps them),
+//   psnr_dec[f]: the same function after SetLogicalBlocks
+//     (codec/dxt_image.cpp:452-455), i.e. over the blocks a decoder rebuilds
+//     from the physical words (both optional: NULL skips them).
 // t_fit is the constructor time minus a separate PNG load of the same file.
 int ref_encode_sequence(const uint8_t* frames, int w, int h, int n_frames, int search_area,
                         int threshold, int intra_interval, const char* tmpdir, uint64_t* phys,
                         uint8_t* motion, uint32_t* unique, int* n_unique, uint8_t* eps,
-                        double* t_fit, double* t_reencode) {
+                        double* t_fit, double* t_reencode, double* psnr_enc, double* psnr_dec) {
   const int bw = (w + 3) >> 2, bh = (h + 3) >> 2, nb = bw * bh;
   const std::string path = std::string(tmpdir) + "/ref_frame.png";
   std::unique_ptr<MPTC::DXTImage> prev_frame(nullptr), curr_frame(nullptr);
@@ -135,6 +141,12 @@ int ref_encode_sequence(const uint8_t* frames, int w, int h, int n_frames, int s
     n_unique[f] = static_cast<int>(curr_frame->_unique_palette.size());
     for (int u = 0; u < n_unique[f]; ++u)
       unique[static_cast<size_t>(f) * nb + u] = curr_frame->_unique_palette[u];
+    if (psnr_enc) psnr_enc[f] = curr_frame->PSNR();
+    if (psnr_dec) {
+      // the next frame's inter search reads only _physical_blocks of this one
+      curr_frame->SetLogicalBlocks();
+      psnr_dec[f] = curr_frame->PSNR();
+    }
     // intra schedule of codec/codec.cpp:1462-1469
     if (curr_intra_num <= static_cast<uint32_t>(intra_interval)) {
       set_intra = false;

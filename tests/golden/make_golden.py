@@ -5,6 +5,7 @@ Run in the build container (where the reference sources are), e.g.
     python tests/golden/make_golden.py units
     python tests/golden/make_golden.py seq c1_s8_t0
     python tests/golden/make_golden.py all-seq      # every sequence, one process each
+    python tests/golden/make_golden.py psnr         # the reference's PSNR of leading frames
 The fixtures are data only: inputs are regenerated from the seeded frame
 generator (mptc_amd/csrc/framegen.h), outputs are the reference's.
 """
@@ -134,12 +135,39 @@ def sequence(name: str) -> None:
     print(name, "done in %.1fs" % wall)
 
 
+# Sequences whose leading frames get the reference's own PSNR values, and how
+# many frames of each (the 1080p ones take ~10 s per frame single-threaded).
+PSNR_FRAMES = {"c1_s8_t0": 8, "c1_s8_t8": 8, "c1_s16_t0": 8, "c2_t0": 3, "c2_t8": 3}
+
+
+def psnr_fixture() -> None:
+    """psnr_ref.json: DXTImage::PSNR (codec/dxt_image.cpp:363-383) of the
+    reference's own encodes, encoder side (its logical blocks right after
+    Reencode) and decoder side (after SetLogicalBlocks, codec/dxt_image.cpp:452),
+    for the leading frames of PSNR_FRAMES, with the SHA of each frame's blocks
+    so that a test can tie the values to the sequence fixtures."""
+    O = oracle.Oracle()
+    R = oracle.Reference()
+    out = {}
+    for name, n in PSNR_FRAMES.items():
+        spec = SEQUENCES[name]
+        frames = O.synth_frames(synth_cfg(spec), n)
+        res = R.encode_sequence(frames, spec["S"], spec["thr"], spec["intra_interval"], psnr=True)
+        out[name] = [dict(frame=f, phys_sha=sha(r["phys"]), psnr_enc=r["psnr_enc"], psnr_dec=r["psnr_dec"])
+                     for f, r in enumerate(res)]
+        print(name, [round(r["psnr_dec"], 4) for r in res], flush=True)
+    with open(os.path.join(HERE, "psnr_ref.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 def main(argv):
     oracle.build_oracle()
     oracle.build_ref()
     what = argv[1] if len(argv) > 1 else "units"
     if what == "units":
         unit_vectors(oracle.Reference(), oracle.Oracle())
+    elif what == "psnr":
+        psnr_fixture()
     elif what == "seq":
         for name in argv[2:]:
             sequence(name)

@@ -80,10 +80,11 @@ def test_batch_of_uneven_streams_matches_reference(orc):
     enc.close()
 
 
-@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune", [(256, 64, 2, 1, 2, 1), (64, 8, 1, 0, 1, 1),
-                                                         (128, 16, 2, 1, 4, 0), (256, 32, 3, 0, 8, 1),
-                                                         (64, 64, 16, 1, 6, 0), (256, 16, 1, 1, 3, 1)])
-def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks, prune):
+@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune,fix", [(256, 64, 2, 1, 2, 1, 0), (64, 8, 1, 0, 1, 1, 1),
+                                                             (128, 16, 2, 1, 4, 0, 2), (256, 32, 3, 0, 8, 1, 0),
+                                                             (64, 64, 16, 1, 6, 0, 1), (256, 16, 1, 1, 3, 1, 2),
+                                                             (256, 8, 2, 0, 4, 1, 1), (256, 8, 2, 0, 4, 1, 2)])
+def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks, prune, fix):
     from mptc_amd import hip, lib
     from oracle import SynthCfg
     sizes = [(160, 96, 11), (96, 64, 12), (256, 128, 13)]
@@ -92,9 +93,9 @@ def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks, prune):
     want = [orc.encode_sequence(fr, S, thr, 3) for fr in frames]
     total = sum((w // 4) * (h // 4) for w, h, _ in sizes)
     enc = lib.Encoder(256, 128, S, thr, max_batch_frames=3, max_batch_blocks=total)
-    enc.set_tuning(wg, tw, pb, graph, chunks, prune)
+    enc.set_tuning(wg, tw, pb, graph, chunks, prune, fix)
     streams = [Stream(hip, fr) for fr in frames]
-    log(f"tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune}")
+    log(f"tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune} fixpoint={fix}")
     res = _encode_streams(enc, streams, 4, 3)
     st = enc.stats()
     exact = st["exact_inter_candidates"] + st["exact_intra_candidates"]
@@ -125,4 +126,68 @@ def test_autotune_picks_a_valid_shape(orc):
     enc.encode_batch_device([st.io(1, False)])
     want = orc.encode_sequence(frames, spec["S"], spec["thr"], spec["intra_interval"])
     compare_results(st.result(1), want[1], "autotuned frame 1")
+    enc.close()
+
+
+@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune,fix", [(256, 8, 2, 0, 4, 1, 1), (64, 64, 1, 1, 1, 0, 0),
+                                                             (128, 32, 3, 1, 8, 1, 2), (256, 8, 2, 0, 4, 1, 0)])
+def test_tunings_at_1080p_match_reference(orc, wg, tw, pb, graph, chunks, prune, fix):
+    """Launch shapes at the 1080p size the bench times (the first one is the
+    shape the bench's autotuner picks there), against the reference's fixture."""
+    from mptc_amd import hip, lib
+    name = "c2_t0"
+    fx = load_seq(name)
+    spec = SEQUENCES[name]
+    n = 4
+    frames = orc.synth_frames(synth_cfg(spec), n)
+    enc = lib.Encoder(spec["width"], spec["height"], spec["S"], spec["thr"])
+    enc.set_tuning(wg, tw, pb, graph, chunks, prune, fix)
+    st = Stream(hip, frames)
+    log(f"1080p tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune} fixpoint={fix}")
+    res = _encode_streams(enc, [st], n, spec["intra_interval"])
+    for f in range(n):
+        compare_frame(res[f][0], fx["frames"][f], f"{name} frame {f}")
+    enc.close()
+
+
+def test_calls_on_alternating_streams(orc):
+    """One context driven from several streams in turn with no host wait
+    between the calls (each frame has its own output buffers): a call on
+    another stream waits for the previous call's work, so no step overwrites
+    the buffers the previous step's commit still reads, and every frame
+    matches the oracle."""
+    from mptc_amd import hip, lib
+    from mptc_amd.lib import next_intra
+    spec = SEQUENCES["c1_s8_t8"]
+    n = 6
+    frames = orc.synth_frames(synth_cfg(spec), n)
+    want = orc.encode_sequence(frames, spec["S"], spec["thr"], 3)
+    w, h = spec["width"], spec["height"]
+    nb = (w // 4) * (h // 4)
+    enc = lib.Encoder(w, h, spec["S"], spec["thr"])
+    sa, sb = hip.Stream(), hip.Stream()
+    rgb = hip.Buffer.from_array(frames)
+    outs = [dict(phys=hip.Buffer(8 * nb), motion=hip.Buffer(2 * nb), unique=hip.Buffer(4 * nb),
+                 n_unique=hip.Buffer(4), eps=hip.Buffer(6 * nb)) for _ in range(n)]
+    set_intra, num = True, 1
+    for f in range(n):
+        handle = [sa.handle, sb.handle, None][f % 3]
+        o = outs[f]
+        io = dict(d_rgb=rgb.offset(f * w * h * 3), width=w, height=h, is_intra=set_intra,
+                  d_ref=None if set_intra else outs[f - 1]["phys"], d_phys=o["phys"], d_motion=o["motion"],
+                  d_unique=o["unique"], d_n_unique=o["n_unique"], d_eps=o["eps"])
+        log(f"alternating streams: frame {f} on {'ab-'[f % 3]}")
+        enc.encode_batch_device([io], stream=handle)
+        set_intra, num = next_intra(set_intra, num, 3)
+    sa.synchronize()
+    sb.synchronize()
+    enc.synchronize()
+    for f in range(n):
+        o = outs[f]
+        k = int(o["n_unique"].download(np.uint32, 1)[0])
+        got = dict(phys=o["phys"].download(np.uint64, nb), motion=o["motion"].download(np.uint8, 2 * nb),
+                   unique=o["unique"].download(np.uint32, k), eps=o["eps"].download(np.uint8, 6 * nb))
+        compare_results(got, want[f], f"alternating streams frame {f}")
+    sa.destroy()
+    sb.destroy()
     enc.close()

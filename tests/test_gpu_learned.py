@@ -341,3 +341,85 @@ def test_fused_topk_matches_network_logits(env, k, S):
             checked += 1
         assert checked > 0
     enc.close()
+
+
+def test_collect_pairs_after_a_step_on_a_caller_stream(env):
+    """Acceptance sets recorded by a step queued on the caller's stream are
+    collected only after that step's work is done: the pairs equal those of
+    the same step run on the context's own stream."""
+    hip, lib = env
+    spec = dict(SEQUENCES["c1_s8_t8"])
+    frames = lib.synth_frames(synth_cfg(spec), 0, 2)
+    S, thr = spec["S"], spec["thr"]
+    got = []
+    for use_caller in (False, True):
+        seq = Seq(hip, frames)
+        enc = lib.Encoder(seq.w, seq.h, S, thr)
+        enc.record_pairs(True)
+        stream = hip.Stream() if use_caller else None
+        enc.encode_frame_device(seq.rgb_at(0), seq.w, seq.h, True, None, seq.phys[0], seq.motion[0], seq.unique,
+                                seq.n_unique, seq.eps[0], None)
+        enc.encode_frame_device(seq.rgb_at(1), seq.w, seq.h, False, seq.phys[0], seq.phys[1], seq.motion[1],
+                                seq.unique, seq.n_unique, seq.eps[1], None,
+                                stream=stream.handle if stream else None)
+        out = []
+        for net in (0, 1):
+            cap = seq.nb
+            px, tgt, eps = hip.Buffer(48 * cap), hip.Buffer(32 * cap), hip.Buffer(6 * cap)
+            cnt = hip.Buffer.from_array(np.zeros(1, np.uint32))
+            enc.collect_pairs(net, px, tgt, eps, cnt, cap)  # no wait on `stream` before this call
+            n = int(cnt.download(np.uint32, 1)[0])
+            out.append((n, px.download(np.uint8, 48 * n), tgt.download(np.uint32, 8 * n),
+                        eps.download(np.uint8, 6 * n)))
+        if stream:
+            stream.synchronize()
+            stream.destroy()
+        enc.close()
+        got.append(out)
+    for net in (0, 1):
+        a, b = got[0][net], got[1][net]
+        assert a[0] == b[0] and a[0] > 0, (net, a[0], b[0])
+        # pairs are appended in an order set by the workgroups: compare as sets of rows
+        ra = sorted(bytes(a[1][48 * i:48 * i + 48]) + a[2][8 * i:8 * i + 8].tobytes() + bytes(a[3][6 * i:6 * i + 6])
+                    for i in range(a[0]))
+        rb = sorted(bytes(b[1][48 * i:48 * i + 48]) + b[2][8 * i:8 * i + 8].tobytes() + bytes(b[3][6 * i:6 * i + 6])
+                    for i in range(b[0]))
+        assert ra == rb, net
+
+
+def test_learned_1080p_with_network_inference(env, orc):
+    """Config 5 at its size: both networks trained briefly on the exhaustive
+    search's pairs of three 1080p frames, then the learned search (mode 2,
+    k = 8) encodes the same three frames with network inference over all
+    129,600 blocks per frame.  Every reuse is checked by the oracle's accept
+    test and as a decoder reads it."""
+    hip, lib = env
+    spec = SEQUENCES["c2_t0"]
+    S, thr = spec["S"], spec["thr"]
+    n = 3
+    frames = lib.synth_frames(synth_cfg(spec), 0, n)
+    enc = lib.Encoder(spec["width"], spec["height"], S, thr)
+    log("1080p learned: collecting pairs")
+    bufs = _collect(enc, lib, hip, frames, S, thr, 1 << 19)
+    log(f"pairs inter {bufs[0]['n']} intra {bufs[1]['n']}")
+    assert bufs[0]["n"] > 1000 and bufs[1]["n"] > 1000
+    for net in (0, 1):
+        enc.net_init(net, 5 + net)
+        b = bufs[net]
+        enc.net_train(net, b["px"], b["tgt"], b["eps"], b["n"], batch=4096, steps=150, lr=2e-3,
+                      seed_weight=1.0, seed=3, lr_final=2e-4)
+    enc.set_search_mode(2, 8)
+    seq = Seq(hip, frames)
+    enc.profile_enable(True)
+    log("1080p learned: encoding with checks")
+    stats = encode(enc, lib, seq, spec["intra_interval"], check=(orc, S, thr))
+    prof = enc.profile_read("net_l3")
+    enc.profile_enable(False)
+    assert prof[0] >= 2 * n - 1  # inference ran: inter (frames 1, 2) and intra (every frame)
+    reused = sum(s["n_inter"] + s["n_intra"] for s in stats)
+    log(f"1080p learned: {reused} reused blocks over {n} frames")
+    assert reused > 0
+    for s in stats:
+        assert s["n_blocks"] == 129600
+        assert s["inter_candidates"] <= 8 * s["n_blocks"] and s["intra_candidates"] <= 8 * s["n_blocks"]
+    enc.close()

@@ -95,12 +95,39 @@ def test_config1_full_arrays(enc, orc):
 
 @pytest.mark.parametrize("name", ["c2_t0", "c2_t8"])
 def test_config2_1080p(enc, orc, name):
-    _run_sequence(enc, orc, name)
+    res = _run_sequence(enc, orc, name)
+    # PSNR of the GPU's frames (oracle formula, pinned in test_oracle.py)
+    # against the reference's DXTImage::PSNR of its own frames
+    import json
+    with open(os.path.join(os.path.dirname(__file__), "golden", "psnr_ref.json")) as fh:
+        want = json.load(fh)[name]
+    spec = SEQUENCES[name]
+    frames = orc.synth_frames(synth_cfg(spec), len(want))
+    for f, wf in enumerate(want):
+        assert orc.psnr_logical(frames[f], res[f]["phys"], res[f]["eps"]) == pytest.approx(wf["psnr_enc"], abs=1e-9)
+        assert orc.psnr_physical(frames[f], res[f]["phys"]) == pytest.approx(wf["psnr_dec"], abs=1e-9)
 
 
 @pytest.mark.parametrize("name", ["c3_t0", "c3_t8", "c3_t32", "c3_t128"])
 def test_config3_4k_s32(enc, orc, name):
     _run_sequence(enc, orc, name)
+
+
+@pytest.mark.parametrize("fix", [0, 2])
+@pytest.mark.parametrize("name", ["c3_t8", "c3_t128"])
+def test_config3_fixpoint_modes(enc, orc, name, fix):
+    """The 4K intra frame at thresholds 8 and 128 reuses nearly every block,
+    mostly in long chains of left neighbours: the host-polled passes (0) and
+    the persistent kernel with chain jumping (2) both end at the reference's
+    words."""
+    keep = enc.get_tuning()
+    enc.set_tuning(keep["wg_size"], keep["tile_w"], keep["pass_batch"], keep["use_graph"], keep["win_chunks"],
+                   keep["prune"], fix)
+    try:
+        _run_sequence(enc, orc, name)
+        log(f"{name} fixpoint {fix}: {enc.stats()['intra_iterations']} passes on the last frame")
+    finally:
+        enc.set_tuning(**keep)
 
 
 @pytest.mark.parametrize("i", range(12))
@@ -160,9 +187,11 @@ def test_constant_and_two_colour_frames(enc, orc):
     _oracle_vs_gpu(enc, orc, frames, 3, 0, 8, "two-colour")
 
 
-def test_gradient_chains(enc, orc):
+@pytest.mark.parametrize("fix", [0, 1, 2])
+def test_gradient_chains(enc, orc, fix):
     # smooth gradients make long chains of intra reuse (each block may copy a
-    # word a neighbour itself copied), exercising the fixpoint iterations
+    # word a neighbour itself copied), exercising the fixpoint iterations in
+    # every fixpoint mode
     h, w = 64, 128
     yy, xx = np.mgrid[0:h, 0:w]
     frames = []
@@ -170,8 +199,15 @@ def test_gradient_chains(enc, orc):
         img = np.stack([(xx * 2 + s) % 256, (yy * 3 + xx // 4) % 256, (xx + yy + 7 * s) % 256], -1)
         frames.append(img)
     frames = np.array(frames, np.uint8)
-    for thr in (0, 2, 8):
-        _oracle_vs_gpu(enc, orc, frames, 6, thr, 8, f"gradient thr={thr}")
+    flat = np.clip(frames.astype(int) // 16 * 16 + 3, 0, 255).astype(np.uint8)  # wide flat steps
+    keep = enc.get_tuning()
+    enc.set_tuning(**dict(keep, fixpoint=fix))
+    try:
+        for thr in (0, 2, 8, 64):
+            _oracle_vs_gpu(enc, orc, frames, 6, thr, 8, f"gradient thr={thr} fixpoint={fix}")
+            _oracle_vs_gpu(enc, orc, flat, 5, thr, 8, f"flat steps thr={thr} fixpoint={fix}")
+    finally:
+        enc.set_tuning(**keep)
 
 
 def test_synthetic_medium_sequences(enc, orc):

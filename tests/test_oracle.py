@@ -176,3 +176,107 @@ def test_reuse_checker_accepts_the_oracle_and_catches_planted_mistakes(orc):
                 with pytest.raises(AssertionError):
                     check_reuse_frame(orc, rgb, own, phys, r["motion"], unique, r["eps"], False, prev, S, thr)
         prev = r["phys"]
+
+
+def _load_psnr_ref():
+    import json
+    import os
+    from tests.helpers import GOLDEN
+    with open(os.path.join(GOLDEN, "psnr_ref.json")) as fh:
+        return json.load(fh)
+
+
+def _np_decode(phys: np.ndarray, eps: np.ndarray | None, w: int, h: int) -> np.ndarray:
+    """A decoder written from the DXT1 format alone (numpy, vectorised), not
+    from either C implementation: 565 endpoints expanded by bit replication;
+    ep1 > ep2 gives the palette (e1, e2, (2e1+e2)/3, (e1+2e2)/3), otherwise
+    (e1, e2, (e1+e2)/2, black); index of pixel i at bits 2i of the high word.
+    With eps, the four-colour blocks take their 8-bit endpoints from eps (the
+    encoder's logical blocks keep the re-fit values off the 565 grid)."""
+    phys = np.asarray(phys, np.uint64)
+    e1 = (phys & np.uint64(0xFFFF)).astype(np.int64)
+    e2 = ((phys >> np.uint64(16)) & np.uint64(0xFFFF)).astype(np.int64)
+    words = (phys >> np.uint64(32)).astype(np.int64)
+
+    def expand(v):
+        r, g, b = (v >> 11) & 31, (v >> 5) & 63, v & 31
+        return np.stack([(r << 3) | (r >> 2), (g << 2) | (g >> 4), (b << 3) | (b >> 2)], -1)
+
+    a, b = expand(e1), expand(e2)
+    four = e1 > e2
+    if eps is not None:
+        ep = np.asarray(eps, np.int64).reshape(-1, 6)
+        a = np.where(four[:, None], ep[:, :3], a)
+        b = np.where(four[:, None], ep[:, 3:], b)
+    p2 = np.where(four[:, None], (2 * a + b) // 3, (a + b) // 2)
+    p3 = np.where(four[:, None], (a + 2 * b) // 3, 0)
+    pal = np.stack([a, b, p2, p3], 1)  # (nb, 4, 3)
+    idx = (words[:, None] >> (2 * np.arange(16))[None, :]) & 3  # (nb, 16)
+    px = np.take_along_axis(pal, idx[:, :, None].repeat(3, 2), 1)  # (nb, 16, 3)
+    bw, bh = w // 4, h // 4
+    return px.reshape(bh, bw, 4, 4, 3).transpose(0, 2, 1, 3, 4).reshape(h, w, 3)
+
+
+def _np_psnr(rgb: np.ndarray, dec: np.ndarray) -> float:
+    # 10 log10(3 * 255^2 / (SSE / (W H))) -- codec/dxt_image.cpp:363-383, Metrics.py:54-66
+    h, w, _ = rgb.shape
+    sse = float(((rgb.astype(np.int64) - dec.astype(np.int64)) ** 2).sum())
+    return 10.0 * np.log10(3.0 * 255.0 * 255.0 / (sse / (w * h)))
+
+
+@pytest.mark.parametrize("name", ["c1_s8_t0", "c1_s8_t8", "c1_s16_t0"])
+def test_psnr_matches_reference(orc, name):
+    """orc_psnr_logical / orc_psnr_physical against DXTImage::PSNR of the
+    compiled reference (encoder side: logical blocks after Reencode; decoder
+    side: after SetLogicalBlocks) on the same frames and blocks."""
+    from tests.helpers import sha
+    want = _load_psnr_ref()[name]
+    spec = SEQUENCES[name]
+    frames = orc.synth_frames(synth_cfg(spec), len(want))
+    res = orc.encode_sequence(frames, spec["S"], spec["thr"], spec["intra_interval"])
+    for f, (r, wf) in enumerate(zip(res, want)):
+        assert sha(r["phys"]) == wf["phys_sha"], f"frame {f}"
+        assert orc.psnr_logical(frames[f], r["phys"], r["eps"]) == pytest.approx(wf["psnr_enc"], abs=1e-9)
+        assert orc.psnr_physical(frames[f], r["phys"]) == pytest.approx(wf["psnr_dec"], abs=1e-9)
+
+
+def test_psnr_closed_form_and_decoder():
+    """The reference's PSNR values, and the oracle's by the previous test,
+    equal the textbook formula over an image decoded by an independent numpy
+    DXT1 decoder.  Mistakes a restatement could make -- ignoring the logical
+    endpoints, a transposed block layout, black in the four-colour palette --
+    each move the value off the reference's."""
+    import os
+    from tests.helpers import GOLDEN
+    import oracle
+    O = oracle.Oracle()
+    want = _load_psnr_ref()["c1_s8_t8"]
+    spec = SEQUENCES["c1_s8_t8"]
+    frames = O.synth_frames(synth_cfg(spec), 3)
+    res = O.encode_sequence(frames, spec["S"], spec["thr"], spec["intra_interval"])
+    w, h = spec["width"], spec["height"]
+    for f in range(3):
+        rgb, r = frames[f], res[f]
+        enc = _np_psnr(rgb, _np_decode(r["phys"], r["eps"], w, h))
+        dec = _np_psnr(rgb, _np_decode(r["phys"], None, w, h))
+        assert enc == pytest.approx(want[f]["psnr_enc"], abs=1e-9)
+        assert dec == pytest.approx(want[f]["psnr_dec"], abs=1e-9)
+        # planted mistakes, each must be visible
+        assert abs(dec - want[f]["psnr_enc"]) > 1e-3  # encoder and decoder sides differ
+        t = _np_decode(r["phys"], None, w, h).reshape(h // 4, 4, w // 4, 4, 3).transpose(0, 3, 2, 1, 4).reshape(h, w, 3)
+        assert abs(_np_psnr(rgb, t) - want[f]["psnr_dec"]) > 1e-3
+        phys_black = np.asarray(r["phys"], np.uint64).copy()
+        four = (phys_black & np.uint64(0xFFFF)) > ((phys_black >> np.uint64(16)) & np.uint64(0xFFFF))
+        swapped = ((phys_black & np.uint64(0xFFFF)) << np.uint64(16)) | ((phys_black >> np.uint64(16)) & np.uint64(0xFFFF)) | (phys_black & ~np.uint64(0xFFFFFFFF))
+        phys_black[four] = swapped[four]  # four-colour blocks read as three-colour
+        assert abs(_np_psnr(rgb, _np_decode(phys_black, None, w, h)) - want[f]["psnr_dec"]) > 1e-3
+    # the full-array fixture of c1_s8_t0 (the reference's own blocks and endpoints)
+    arr = np.load(os.path.join(GOLDEN, "seq_c1_s8_t0.npz"))
+    want0 = _load_psnr_ref()["c1_s8_t0"]
+    spec0 = SEQUENCES["c1_s8_t0"]
+    frames0 = O.synth_frames(synth_cfg(spec0), 2)
+    for f in range(2):
+        assert _np_psnr(frames0[f], _np_decode(arr["phys"][f], arr["eps"][f], 256, 256)) == pytest.approx(
+            want0[f]["psnr_enc"], abs=1e-9)
+        assert _np_psnr(frames0[f], _np_decode(arr["phys"][f], None, 256, 256)) == pytest.approx(
+            want0[f]["psnr_dec"], abs=1e-9)

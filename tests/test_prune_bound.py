@@ -156,3 +156,64 @@ def test_planted_bound_mistakes_are_caught(orc, variant):
             else:
                 caught += _w_variant(px48, c, variant) >= 48 * (thr + eo + 1) + 64
     assert caught > 0, variant
+
+
+# ---- E and passing-set offsets of the pruned intra search --------------------
+
+K_PRUNE_BATCH = 16  # kPruneBatch (prune_kernels.h)
+K_PASS_WORDS = 8    # kPassWords (batch_kernels.h)
+K_MAX_CHUNKS = 8    # kMaxChunks (mptc_hip.hip)
+
+
+def chunks_for(S: int, want: int) -> int:
+    """mptc_hip.hip chunks_for: the largest chunk count up to `want` that
+    divides the 2S window rows and keeps every chunk a whole number of
+    four-position E groups."""
+    for k in range(min(want, K_MAX_CHUNKS), 1, -1):
+        if (2 * S) % k == 0 and ((2 * S // k) * 2 * S) % 4 == 0:
+            return k
+    return 1
+
+
+def pruned_intra_writes(S: int, n_chunks: int):
+    """Every (E group, passing-set word) the pruned intra search writes for one
+    block, following k_pruned_search<true>'s loops (prune_kernels.h): chunk c
+    scans positions [p0, p1) in batches of kPruneBatch; each batch stores
+    E4[(pb + k) >> 2] for k = 0, 4, .. < nb and ORs its bits into passing-set
+    word pb >> 5 and, when the batch straddles a 32-bit boundary, the next."""
+    W2 = 2 * S
+    rows = W2 // n_chunks
+    groups, words = [], []
+    for c in range(n_chunks):
+        p0, p1 = c * rows * W2, (c + 1) * rows * W2
+        for pb in range(p0, p1, K_PRUNE_BATCH):
+            nb = min(K_PRUNE_BATCH, p1 - pb)
+            assert nb % 4 == 0, (S, n_chunks, pb, nb)  # the kernel packs four entries per store
+            groups += [(pb + k) >> 2 for k in range(0, nb, 4)]
+            words.append(pb >> 5)
+            sh = pb & 31
+            if sh > 16 and pb + nb > ((pb >> 5) + 1) * 32:
+                words.append((pb >> 5) + 1)
+    return groups, words
+
+
+@pytest.mark.parametrize("S", range(1, 9))
+def test_pruned_search_offsets_stay_in_bounds(S):
+    """The E groups and passing-set words the pruned intra search writes,
+    for every window-chunk setting the tuning allows, lie inside the buffers
+    the context allocates (S^2 groups of four int16 deltas per block,
+    kPassWords words per block) and cover every window position exactly once.
+    The first, uncommitted version of this kernel faulted the GPU with an E
+    store past its group; this pins the index arithmetic on the CPU.  The
+    re-evaluation pass writes slot s of group s >> 2 for s < 4S^2, inside the
+    same S^2 groups."""
+    for want in range(1, K_MAX_CHUNKS + 1):
+        K = chunks_for(S, want)
+        groups, words = pruned_intra_writes(S, K)
+        assert max(groups) < S * S and min(groups) >= 0, (S, K)
+        assert sorted(groups) == list(range(S * S)), (S, K)  # each group once
+        assert max(words) < K_PASS_WORDS and (4 * S * S - 1) >> 5 < K_PASS_WORDS, (S, K)
+    # a planted off-by-one (a batch that writes one group past its end) is caught
+    groups, _ = pruned_intra_writes(S, 1)
+    bad = groups + [max(groups) + 1]
+    assert max(bad) >= S * S
