@@ -153,8 +153,7 @@ def cpu_baseline(spec, steps):
 
 KERNELS = ("fit", "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine", "intra_apply",
            "intra_reeval", "intra_select", "commit", "unique", "net_input", "net_l12", "net_l3",
-           "propose_inter", "top_intra", "lintra_pass", "pruned_inter", "pruned_intra", "fold_intra",
-           "intra_fixpoint")
+           "propose_inter", "top_intra", "lintra_pass", "pruned_inter", "pruned_intra", "fold_intra")
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "encode_1080p_plain_hbm_traffic.md")

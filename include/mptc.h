@@ -61,13 +61,6 @@ typedef struct {
   int32_t prune;      /* 1: for S <= 8 and thresholds 0..4, evaluate exactly only the
                          candidates an error lower bound cannot rule out (same output);
                          0: evaluate every candidate */
-  int32_t fixpoint;   /* intra fixpoint of the exhaustive search: 1 = one persistent
-                         (cooperative) launch runs every pass and the call returns as
-                         soon as the step is queued; 2 = the same, each pass also
-                         handing every chain of reusing blocks its root's word at once
-                         (for long reuse chains: large windows, high thresholds);
-                         0 = passes queued in batches of pass_batch, the host checking
-                         convergence between batches */
 } mptc_tuning;
 
 /* One frame of a batched step: device pointers, layouts as mptc_reencode. */
@@ -133,10 +126,7 @@ int32_t mptc_reencode(mptc_ctx* ctx, const uint8_t* rgb, int32_t width, int32_t 
  * d_eps_out and d_kind_out may be NULL.  Work is queued on `hip_stream`
  * (a hipStream_t; NULL = the context's stream); the call returns once the
  * frame's work is queued and its intra fixpoint is reached (it synchronises
- * with the stream to test convergence; with the persistent fixpoint, the
- * default, it returns as soon as the step is queued).  Errors of queued work
- * (a fixpoint that did not settle) are reported by the context's next call,
- * mptc_synchronize or mptc_get_stats.  Same layouts as mptc_reencode.
+ * with the stream to test convergence).  Same layouts as mptc_reencode.
  * Successive calls on one context may use different streams: a call on
  * another stream than the previous call's first waits for that call's work
  * (the context's buffers are shared), so outputs never race. */
@@ -155,8 +145,7 @@ int32_t mptc_encode_batch_device(mptc_ctx* ctx, int32_t n_frames, const mptc_fra
                                  void* hip_stream);
 
 /* Launch-shape tuning.  mptc_autotune times every candidate shape (wg_size x
- * tile_w x win_chunks x use_graph, prune 0/1 where the pruned search applies and
- * fixpoint 0/1/2 in the exhaustive search) on
+ * tile_w x win_chunks x use_graph, and prune 0/1 where the pruned search applies) on
  * `steps` encodes of the given batch (outputs are overwritten), keeps the
  * fastest and reports it. */
 int32_t mptc_get_tuning(mptc_ctx* ctx, mptc_tuning* out);
@@ -181,8 +170,7 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
  * "intra_apply", "intra_reeval", "intra_select", "commit", "unique";
  * learned search: "net_input", "net_l12", "net_l3" (inference), "propose_inter",
  * "top_intra", "lintra_pass"; "net_forward": training products; pruned search:
- * "pruned_inter", "pruned_intra", "fold_intra"; the persistent fixpoint:
- * "intra_fixpoint") since it was enabled
+ * "pruned_inter", "pruned_intra", "fold_intra") since it was enabled
  * (synchronises).  While enabled, steps run as individual launches, not graphs. */
 int32_t mptc_profile_enable(mptc_ctx* ctx, int32_t enable);
 int32_t mptc_profile_read(mptc_ctx* ctx, const char* name, int64_t* launches, double* total_ms);

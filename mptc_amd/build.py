@@ -30,7 +30,9 @@ FLAGS = [
 
 HOST_DIR = os.path.join(HERE, "host")
 HOST_LIB = os.path.join(HOST_DIR, "libmptc_host.so")
+CODEC_LIB = os.path.join(HOST_DIR, "libmptc_codec.so")
 HOST_BIN = os.path.join(HOST_DIR, "encode_seq")
+ENCODE_BIN = os.path.join(HOST_DIR, "mptc_encode")
 CXX = os.environ.get("CXX", "g++")
 
 
@@ -43,20 +45,34 @@ def _run(cmd, verbose):
         raise RuntimeError(f"{cmd[0]} failed")
 
 
+def build_codec(verbose: bool = False) -> str:
+    """The CPU entropy stage and file format (mptc_amd/host/codec.cpp,
+    include/mptc_codec.h): plain C++, no GPU dependency."""
+    inc = os.path.join(ROOT, "include")
+    tmp = f"{CODEC_LIB}.tmp{os.getpid()}"
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", inc,
+          os.path.join(HOST_DIR, "codec.cpp"), "-o", tmp], verbose)
+    os.replace(tmp, CODEC_LIB)
+    return CODEC_LIB
+
+
 def build_host(verbose: bool = False) -> str:
     """The C++ host layer (mptc_amd/host) over libmptc_hip.so, and its
     sequence driver; both find libmptc_hip.so through their run path."""
+    build_codec(verbose)
     inc = os.path.join(ROOT, "include")
     tmp = f"{HOST_LIB}.tmp{os.getpid()}"
     _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", inc,
-          os.path.join(HOST_DIR, "mptc_host.cpp"), "-L", HERE, "-lmptc_hip",
-          "-Wl,-rpath,$ORIGIN/..", "-o", tmp], verbose)
+          os.path.join(HOST_DIR, "mptc_host.cpp"), os.path.join(HOST_DIR, "png.cpp"),
+          "-L", HOST_DIR, "-lmptc_codec", "-L", HERE, "-lmptc_hip", "-lz",
+          "-Wl,-rpath,$ORIGIN:$ORIGIN/..", "-o", tmp], verbose)
     os.replace(tmp, HOST_LIB)
-    tmpb = f"{HOST_BIN}.tmp{os.getpid()}"
-    _run([CXX, "-O2", "-std=c++17", "-Wall", "-I", inc, os.path.join(HOST_DIR, "encode_seq.cpp"),
-          "-L", HOST_DIR, "-lmptc_host", "-L", HERE, "-lmptc_hip",
-          "-Wl,-rpath,$ORIGIN:$ORIGIN/..", "-o", tmpb], verbose)
-    os.replace(tmpb, HOST_BIN)
+    for src, out in (("encode_seq.cpp", HOST_BIN), ("mptc_encode.cpp", ENCODE_BIN)):
+        tmpb = f"{out}.tmp{os.getpid()}"
+        _run([CXX, "-O2", "-std=c++17", "-Wall", "-I", inc, os.path.join(HOST_DIR, src),
+              "-L", HOST_DIR, "-lmptc_host", "-lmptc_codec", "-L", HERE, "-lmptc_hip",
+              "-Wl,-rpath,$ORIGIN:$ORIGIN/..", "-o", tmpb], verbose)
+        os.replace(tmpb, out)
     return HOST_LIB
 
 

@@ -66,9 +66,6 @@ struct DevStats {
   unsigned int n_unique, active_passes;  // passes that changed a word
   unsigned int n_changed[2];             // length of the changed-word list, by pass parity
   unsigned int pass_done;                // workgroups of the running fixpoint pass that finished
-  unsigned int bar[2];                   // grid barrier of the persistent fixpoint: arrivals, generation
-  unsigned int fix_error;                // persistent fixpoint: 1 barrier time-out, 2 not converged
-  unsigned int jump_chg[3];              // persistent fixpoint, chain jumping: pointers moved, by round % 3
 };
 
 // The pruning bound's table of an index word (prune_kernels.h).
@@ -768,201 +765,6 @@ __global__ void __launch_bounds__(256) k_intra_select(const FrameDev* __restrict
   const bool changed = k < total && select_block(fr, sid, k, S, thr, own, cw, ctab, E4, nbs, dirty, dec, pm);
   wg_append(changed, k, chg_list, &st->n_changed[par], &s_base);
   pass_end(st, par, nullptr, conv);
-}
-
-// ---------------------------------------------------------------------------
-// The whole fixpoint in one persistent launch.
-// ---------------------------------------------------------------------------
-
-// Longest a workgroup waits at a grid barrier: 4 s of the 100 MHz real-time
-// counter (a phase's legitimate work is well under 1 s at every size the
-// context admits).
-constexpr unsigned long long kBarrierTicks = 400000000ull;
-
-// Grid-wide barrier of a kernel whose workgroups are all resident (launched
-// cooperatively).  bar[0] counts arrivals, bar[1] is the generation.  Every
-// thread releases its writes at device scope before and acquires after, so a
-// phase's plain stores are visible to every workgroup of the next phase, on
-// any XCD.  The wait is bounded: a workgroup that waits longer than
-// kBarrierTicks, or sees *err set, sets *err and gets false; the caller then
-// leaves its loop, and every other workgroup times out or sees *err at this or
-// its next barrier, so the grid drains.
-__device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int* err, unsigned int* s_ok) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned ok = 1u;
-    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived + 1u == gridDim.x) {
-      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-            __builtin_amdgcn_s_memrealtime() - t0 > kBarrierTicks) {
-          atomicOr(err, 1u);
-          ok = 0u;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-    }
-    *s_ok = ok;
-  }
-  __syncthreads();
-  __threadfence();
-  return *s_ok != 0u;
-}
-
-// The block an accepted intra decision of block k (global index) reuses.
-__device__ __forceinline__ int intra_source(const FrameDev& d, int k, uint64_t dc, int S) {
-  const int kl = k - d.base, bx = kl % d.bw, by = kl / d.bw;
-  const int i = bx + (int)((dc >> 32) & 0xFFu) - S, j = by + (int)((dc >> 40) & 0xFFu) - 2 * S + 1;
-  return d.base + j * d.bw + i;
-}
-
-// All fixpoint passes of a step in one cooperative launch: pass p re-evaluates
-// the pairs that see a word of changed list p & 1 (cleared list (p & 1) ^ 1
-// first), then re-folds the flagged blocks into list (p & 1) ^ 1, until a pass
-// changes no word.  The phases are separated by grid barriers, so every
-// workgroup reads the same list lengths and takes the same branches.  The loop
-// ends after max_passes passes at most (st->fix_error bit 1: not converged) or
-// on a barrier time-out (bit 0).  Reports into the same counters as the
-// per-pass kernels.
-//
-// Chain jumping (jump != 0).  A block that reuses a word takes its source's
-// word, and in flat content whole rows of blocks reuse their left
-// neighbour's: a word then travels one block per pass, and the words of a
-// chain change at every pass until the chain settles.  With jumping, a pass
-// first re-folds the flagged blocks (decisions only), then follows each
-// reusing block's chain of sources to its root by pointer doubling (anc:
-// 2 x nbs ints, ping-pong), and every block takes its root's word at once:
-// the root's own word if the root searched and reused nothing, else the
-// root's word, which the intra search does not change.  The loop still ends
-// only when a pass changes no word; there every block's word equals the word
-// its decision reuses (its source shares its root), its decision is folded
-// from deltas of the current words, so the words are the unique fixpoint of
-// the raster-triangular system, whatever path led there.
-__global__ void __launch_bounds__(256) k_intra_fixpoint(const FrameDev* __restrict__ fr,
-                                                        const uint8_t* __restrict__ sid, int total, int S, int thr,
-                                                        const uint64_t* __restrict__ own,
-                                                        const uint8_t* __restrict__ need, uint32_t* __restrict__ cw,
-                                                        float4* __restrict__ ctab, int* __restrict__ chg_list,
-                                                        uint2* __restrict__ E4, size_t nbs,
-                                                        uint32_t* __restrict__ dirty, const int* __restrict__ orig,
-                                                        uint64_t* __restrict__ dec, DevStats* __restrict__ st,
-                                                        uint32_t* __restrict__ pm, int max_passes,
-                                                        unsigned int* __restrict__ err_out, int jump,
-                                                        int* __restrict__ anc) {
-  __shared__ uint8_t s_snap[512];
-  __shared__ float2 s_ab[4];
-  __shared__ unsigned long long s_acc;
-  __shared__ unsigned int s_base, s_ok;
-  const int lane = (int)(threadIdx.x & 63u);
-  load_lds_tables(s_snap, s_ab);
-  const size_t gthreads = (size_t)gridDim.x * blockDim.x;
-  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long n_eval = 0;
-  int p = 0;
-  for (;; ++p) {
-    const int par = p & 1;
-    // every workgroup reads the list length after the barrier that ended its
-    // writes; nothing writes it again before the next barrier
-    const unsigned n_chg = __hip_atomic_load(&st->n_changed[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (n_chg == 0u) break;  // the previous pass changed nothing: fixpoint
-    if (p >= max_passes) {
-      if (gid == 0) atomicOr(&st->fix_error, 2u);
-      break;
-    }
-    if (gid == 0) {
-      st->n_changed[par ^ 1] = 0u;
-      atomicAdd(&st->active_passes, 1u);
-    }
-    n_eval += reeval_items(fr, sid, S, thr, own, need, cw, ctab, chg_list, n_chg, E4, nbs, dirty, orig, dec, pm,
-                           s_snap, s_ab, gid, gthreads);
-    if (!grid_barrier(&st->bar[0], &st->fix_error, &s_ok)) break;
-    if (jump) {
-      // decisions of the flagged blocks
-      for (size_t k = gid; k < (size_t)total; k += gthreads) {
-        uint64_t dc;
-        (void)decide_block(fr, sid, (int)k, S, thr, cw, E4, nbs, dirty, dec, pm, dc);
-      }
-      if (gid == 0) st->jump_chg[0] = 0u;
-      if (!grid_barrier(&st->bar[0], &st->fix_error, &s_ok)) break;
-      // parents: the reused block, or the block itself
-      for (size_t k = gid; k < (size_t)total; k += gthreads) {
-        int a = (int)k;
-        if (need[k]) {
-          const uint64_t dc = dec[k];
-          if (dec_acc(dc)) a = intra_source(fr[sid[k]], (int)k, dc, S);
-        }
-        anc[k] = a;
-      }
-      if (!grid_barrier(&st->bar[0], &st->fix_error, &s_ok)) break;
-      // pointer doubling until every block points at its root; round r
-      // counts its changes in jump_chg[r % 3] and clears the counter of round
-      // r + 1, which no workgroup reads any more (read before barrier r - 1)
-      int* cur = anc;
-      int* nxt = anc + nbs;
-      bool ok = true;
-      for (int r = 0; r < 32; ++r) {
-        if (gid == 0) st->jump_chg[(r + 1) % 3] = 0u;
-        bool moved = false;
-        for (size_t k = gid; k < (size_t)total; k += gthreads) {
-          const int a = cur[k];
-          const int b = cur[a];
-          nxt[k] = b;
-          moved |= b != a;
-        }
-        if (__ballot(moved) != 0ull && lane == 0) atomicOr(&st->jump_chg[r % 3], 1u);
-        ok = grid_barrier(&st->bar[0], &st->fix_error, &s_ok);
-        if (!ok) break;
-        int* t = cur;
-        cur = nxt;
-        nxt = t;
-        if (__hip_atomic_load(&st->jump_chg[r % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) break;
-      }
-      if (!ok) break;
-      // every block takes its root's word
-      for (size_t k0 = (size_t)blockIdx.x * blockDim.x; k0 < (size_t)total; k0 += gthreads) {
-        const int k = (int)(k0 + threadIdx.x);
-        bool changed = false;
-        if (k < total && need[k]) {
-          const int rt = cur[k];
-          const uint32_t w = need[rt] ? (uint32_t)(own[rt] >> 32) : cw[rt];
-          if (w != cw[k]) {
-            cw[k] = w;
-            const CandTab t = cand_table(w);
-            ctab[k] = make_float4(t.asq, t.bsq, t.ab, t.f);
-            changed = true;
-          }
-        }
-        wg_append(changed, k, chg_list, &st->n_changed[par ^ 1], &s_base);
-      }
-      if (!grid_barrier(&st->bar[0], &st->fix_error, &s_ok)) break;
-      continue;
-    }
-    // part 2 over every block: the loop count is the same for all threads of
-    // a workgroup (wg_append needs them all)
-    for (size_t k0 = (size_t)blockIdx.x * blockDim.x; k0 < (size_t)total; k0 += gthreads) {
-      const int k = (int)(k0 + threadIdx.x);
-      const bool changed = k < total && select_block(fr, sid, k, S, thr, own, cw, ctab, E4, nbs, dirty, dec, pm);
-      wg_append(changed, k, chg_list, &st->n_changed[par ^ 1], &s_base);
-    }
-    if (!grid_barrier(&st->bar[0], &st->fix_error, &s_ok)) break;
-  }
-  wg_count(&s_acc, n_eval, st->reeval_cands);
-  // errors go to err_out (host-visible memory the caller checks later; racing
-  // writers all write a non-zero value)
-  if (threadIdx.x == 0) {
-    const unsigned e = __hip_atomic_load(&st->fix_error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (e && err_out) {
-      __hip_atomic_fetch_or(err_out, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      __threadfence_system();
-    }
-  }
 }
 
 // Which intra offsets pass the accept test at the fixpoint (the training

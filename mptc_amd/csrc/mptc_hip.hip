@@ -106,14 +106,14 @@ int init_tables(int device) {
 enum KernelId {
   KF_FIT, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
   KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
-  KF_NET_L3, KF_PRUNED_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_FIXPOINT, KF_N
+  KF_NET_L3, KF_PRUNED_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
 };
 const char* kKernelNames[KF_N] = {"fit",          "inter",
                                   "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
                                   "intra_apply",  "intra_reeval", "intra_select", "commit",
                                   "unique",       "net_input",   "net_forward", "propose_inter",
                                   "top_intra",    "lintra_pass",  "net_l12",     "net_l3",
-                                  "pruned_inter", "pruned_intra", "fold_intra", "intra_fixpoint"};
+                                  "pruned_inter", "pruned_intra", "fold_intra"};
 constexpr int kMaxChunks = 8;  // window-row chunks per block in the search kernels
 
 constexpr int kReevalGrid = 4096;  // workgroups of the re-evaluation pass (grid-stride)
@@ -180,7 +180,7 @@ struct mptc_ctx {
   // work on the context's buffers is ordered whatever streams callers use.
   hipEvent_t done_ev = nullptr;
   hipStream_t done_stream = nullptr;  // the stream done_ev was last recorded on (null: never)
-  mptc_tuning tune{256, 16, 2, 0, 4, 1, 1};  // the fastest 1080p shape measured on MI355X
+  mptc_tuning tune{256, 16, 2, 0, 4, 1};  // the fastest 1080p shape measured on MI355X
   // device buffers (one entry per block of the batch unless noted)
   FrameDev* d_fr = nullptr;  // kMaxFrames descriptors
   FrameDev* h_fr = nullptr;  // pinned staging ring: 4 x kMaxFrames
@@ -190,7 +190,6 @@ struct mptc_ctx {
   uint32_t *d_rw = nullptr, *d_cw = nullptr, *d_cwn = nullptr, *d_dirty = nullptr, *d_tiles = nullptr;
   float4 *d_rtab = nullptr, *d_ctab = nullptr;
   int *d_orig = nullptr, *d_chg = nullptr;
-  int* d_anc = nullptr;  // chain jumping of the persistent fixpoint: 2 x nbs pointers
   uint2* d_E = nullptr;  // S^2 groups of four int16 error deltas per block
   uint4* d_summ = nullptr;  // kMaxChunks selection summaries per block
   DevStats* d_stats = nullptr;
@@ -205,8 +204,7 @@ struct mptc_ctx {
   uint32_t *d_unique = nullptr, *d_nunique = nullptr;
   // batch shape the maps and graphs were built for
   std::vector<int> shape;
-  mptc_tuning shape_tune{0, 0, 0, 0, 0, 0, 0};
-  int fix_grid = 0;  // workgroups of the persistent fixpoint launch (all resident)
+  mptc_tuning shape_tune{0, 0, 0, 0, 0, 0};
   int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0, n_ltiles = 0;
   GraphSet graphs;
   bool any_inter = true;  // the batch holds an inter frame
@@ -304,7 +302,6 @@ int check_tuning(const mptc_tuning& t) {
   if (t.pass_batch < 1 || t.pass_batch > 16) return fail(MPTC_EINVAL, "pass_batch must be in 1..16");
   if (t.win_chunks < 1 || t.win_chunks > kMaxChunks) return fail(MPTC_EINVAL, "win_chunks must be in 1..8");
   if (t.prune != 0 && t.prune != 1) return fail(MPTC_EINVAL, "prune must be 0 or 1");
-  if (t.fixpoint < 0 || t.fixpoint > 2) return fail(MPTC_EINVAL, "fixpoint must be 0, 1 or 2");
   return MPTC_OK;
 }
 
@@ -843,92 +840,11 @@ int capture(hipStream_t s, hipGraphExec_t* exec, Q&& q) {
   return MPTC_OK;
 }
 
-// Sticky error of queued work: the persistent fixpoint kernel sets h_conv[1]
-// when a grid barrier timed out or the passes did not converge.
-int check_async_error(const mptc_ctx* c) {
-  const uint32_t e = *reinterpret_cast<volatile const uint32_t*>(c->h_conv + 1);
-  if (e & 1u) return fail(MPTC_EHIP, "intra fixpoint kernel: a grid barrier timed out");
-  if (e & 2u) return fail(MPTC_ESTATE, "intra fixpoint did not converge");
-  return MPTC_OK;
-}
-
-// Workgroups per compute unit of the persistent fixpoint launch (capped by
-// its occupancy): more hide the re-evaluations' latency, fewer make the grid
-// barriers cheaper.
-#ifndef MPTC_FIX_GROUPS_PER_CU
-#define MPTC_FIX_GROUPS_PER_CU 4
-#endif
-constexpr int kFixGroupsPerCu = MPTC_FIX_GROUPS_PER_CU;
-
-// All fixpoint passes in one cooperative launch (k_intra_fixpoint): every
-// workgroup resident, grid barriers between the phases.
-int queue_fixpoint(mptc_ctx* c, hipStream_t s) {
-  if (c->fix_grid == 0) {
-    int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_intra_fixpoint),
-                                                         256, 0));
-    if (per_cu < 1) return fail(MPTC_EHIP, "persistent fixpoint kernel does not fit a compute unit");
-    c->fix_grid = c->n_cu * std::min(per_cu, kFixGroupsPerCu);
-  }
-  const FrameDev* fr = c->d_fr;
-  const uint8_t* sid = c->d_sid;
-  int total = c->total, S = c->cfg.search_area, thr = c->cfg.err_threshold;
-  const uint64_t* own = c->d_own;
-  const uint8_t* need = c->d_need;
-  uint32_t* cw = c->d_cw;
-  float4* ctab = c->d_ctab;
-  int* chg = c->d_chg;
-  uint2* E4 = c->d_E;
-  size_t nbs = c->nbs;
-  uint32_t* dirty = c->d_dirty;
-  const int* orig = c->d_orig;
-  uint64_t* dec = c->d_intra;
-  DevStats* st = c->d_stats;
-  uint32_t* pm = use_prune(c) ? c->d_pm : nullptr;
-  int max_passes = total + 8;  // the raster-triangular system settles within one pass per block
-  uint32_t* err_out = c->d_conv + 1;
-  int jump = c->tune.fixpoint == 2 ? 1 : 0;
-  int* anc = c->d_anc;
-  void* args[] = {&fr, &sid, &total, &S, &thr, &own, &need, &cw, &ctab, &chg, &E4, &nbs, &dirty, &orig, &dec, &st,
-                  &pm, &max_passes, &err_out, &jump, &anc};
-  hipError_t ce = hipSuccess;
-  int rc = launch(c, s, KF_FIXPOINT, [&] {
-    ce = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_intra_fixpoint), dim3(c->fix_grid), dim3(256),
-                                    args, 0, s);
-  });
-  if (ce != hipSuccess) return fail(MPTC_EHIP, std::string("hipLaunchCooperativeKernel: ") + hipGetErrorString(ce));
-  return rc;
-}
-
 // One encode step over the prepared batch.
 int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
   const bool graphs = c->tune.use_graph && !c->prof && fit;
   Graphs& gr = c->graphs.v[c->any_inter ? 1 : 0];
   int rc;
-  if ((rc = check_async_error(c))) return rc;
-  if (c->tune.fixpoint && c->mode == MPTC_SEARCH_EXHAUSTIVE) {
-    // prefix, every fixpoint pass in one persistent launch, suffix: no host
-    // wait; the call returns once the step is queued
-    if (graphs && !gr.prefix) {
-      if ((rc = capture(s, &gr.prefix, [&] { return queue_prefix(c, s, true); }))) return rc;
-      if ((rc = capture(s, &gr.suffix, [&] { return queue_suffix(c, s); }))) return rc;
-    }
-    if (graphs) {
-      HIP_TRY(hipGraphLaunch(gr.prefix, s));
-    } else if ((rc = queue_prefix(c, s, fit))) {
-      return rc;
-    }
-    if ((rc = queue_fixpoint(c, s))) return rc;
-    if (graphs) {
-      HIP_TRY(hipGraphLaunch(gr.suffix, s));
-    } else if ((rc = queue_suffix(c, s))) {
-      return rc;
-    }
-    c->last_passes = -1;
-    c->last_total = c->total;
-    c->last_recorded = c->record;
-    return MPTC_OK;
-  }
   // Passes that change words write their count to h_conv; a step whose first
   // search changed nothing has no such pass (its passes return at once).  No
   // pass of an earlier step can still write it: they ended at a zero count,
@@ -1061,7 +977,6 @@ int32_t mptc_create(const mptc_config* cfg, mptc_ctx** out) {
       (rc = dalloc(&c->d_rw, nb)) || (rc = dalloc(&c->d_cw, nb)) || (rc = dalloc(&c->d_cwn, nb)) ||
       (rc = dalloc(&c->d_dirty, nb)) || (rc = dalloc(&c->d_tiles, max_tiles)) || (rc = dalloc(&c->d_rtab, nb)) ||
       (rc = dalloc(&c->d_ctab, nb)) || (rc = dalloc(&c->d_orig, nb)) || (rc = dalloc(&c->d_chg, nb)) ||
-      (rc = dalloc(&c->d_anc, 2 * c->nbs)) ||
       (rc = dalloc(&c->d_E, groups * c->nbs)) || (rc = dalloc(&c->d_summ, kMaxChunks * c->nbs)) || (rc = dalloc(&c->d_stats, 1)) ||
       (rc = dalloc(&c->d_rgb, one_nb * 48)) || (rc = dalloc(&c->d_ref, one_nb)) || (rc = dalloc(&c->d_phys, one_nb)) ||
       (rc = dalloc(&c->d_motion, one_nb * 2)) || (rc = dalloc(&c->d_eps, one_nb * 6)) ||
@@ -1092,7 +1007,7 @@ void mptc_destroy(mptc_ctx* c) {
   for (hipEvent_t e : c->batch_ev) (void)hipEventDestroy(e);
   void* bufs[] = {c->d_fr,   c->d_sid,   c->d_tsid,   c->d_need,  c->d_own,   c->d_inter,  c->d_intra,
                   c->d_rw,   c->d_cw,    c->d_cwn,    c->d_dirty, c->d_tiles, c->d_rtab,   c->d_ctab,
-                  c->d_orig, c->d_chg,   c->d_E, c->d_summ,     c->d_stats, c->d_rgb,    c->d_ref, c->d_anc,
+                  c->d_orig, c->d_chg,   c->d_E, c->d_summ,     c->d_stats, c->d_rgb,    c->d_ref,
                   c->d_phys, c->d_motion, c->d_eps,   c->d_kind,  c->d_unique, c->d_nunique};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
@@ -1228,15 +1143,12 @@ int32_t mptc_autotune(mptc_ctx* c, int32_t n, const mptc_frame_io* frames, int32
   probe.prune = 1;
   c->tune = probe;
   const int n_prune = use_prune(c) ? 2 : 1;
-  const int n_fix = c->mode == MPTC_SEARCH_EXHAUSTIVE ? 3 : 1;  // the persistent fixpoint applies there
-  for (int fx = 0; fx < n_fix; ++fx)
   for (int pr = 0; pr < n_prune; ++pr)
   for (int graph = 0; graph < 2; ++graph)
   for (int wg : wgs)
     for (int tw : tws)
       for (int kc : chunks) {
-        c->tune = mptc_tuning{wg, tw, keep.pass_batch, graph, kc, n_prune == 2 ? pr : keep.prune,
-                              n_fix == 3 ? (fx + 1) % 3 : keep.fixpoint};
+        c->tune = mptc_tuning{wg, tw, keep.pass_batch, graph, kc, n_prune == 2 ? pr : keep.prune};
         int rc = mptc_encode_batch_device(c, n, frames, nullptr);  // warm-up: maps and graphs
         if (rc) {
           c->tune = keep;
@@ -1268,15 +1180,13 @@ int32_t mptc_synchronize(mptc_ctx* c) {
   HIP_TRY(hipSetDevice(c->cfg.device));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (c->done_stream) HIP_TRY(hipEventSynchronize(c->done_ev));  // the last call's stream
-  return check_async_error(c);
+  return MPTC_OK;
 }
 
 int32_t mptc_get_stats(mptc_ctx* c, mptc_frame_stats* out) {
   if (!c || !out) return fail(MPTC_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(c->cfg.device));
   HIP_TRY(hipDeviceSynchronize());
-  int rc = check_async_error(c);
-  if (rc) return rc;
   DevStats d;
   HIP_TRY(hipMemcpy(&d, c->d_stats, sizeof(d), hipMemcpyDeviceToHost));
   out->n_blocks = c->last_total;

@@ -1,6 +1,14 @@
 // C++ host side of the MPTC encode path (see mptc_host.h).
 #include "mptc_host.h"
 
+#include <dirent.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/mptc_codec.h"
+
 namespace mptc {
 namespace host {
 
@@ -83,6 +91,58 @@ const EncodedFrame& SequenceEncoder::Add(const uint8_t* rgb, int width, int heig
     }
   }
   return prev_;
+}
+
+DirectoryResult CompressDirectory(const std::string& dir, const std::string& out_file, const Options& opt_in,
+                                  int unique_interval) {
+  // every entry but . and .., sorted by name (codec/codec.cpp:1318-1328)
+  DIR* d = opendir(dir.c_str());
+  if (!d) throw Error(MPTC_EINVAL, "mptc: cannot open directory " + dir);
+  std::vector<std::string> names;
+  while (dirent* e = readdir(d)) {
+    if (std::strcmp(e->d_name, ".") == 0 || std::strcmp(e->d_name, "..") == 0) continue;
+    names.push_back(e->d_name);
+  }
+  closedir(d);
+  std::sort(names.begin(), names.end());
+  if (names.empty()) throw Error(MPTC_EINVAL, "mptc: no frames in " + dir);
+  int w = 0, h = 0;
+  std::vector<uint8_t> rgb = LoadPNG(dir + "/" + names[0], &w, &h);
+  Options opt = opt_in;
+  opt.max_width = w;
+  opt.max_height = h;
+  SequenceEncoder seq(opt);
+  mptc_writer* wr = nullptr;
+  if (mptc_writer_create(w, h, opt.search_area, unique_interval, (int)names.size(), &wr) != MPTC_OK)
+    throw Error(MPTC_EINVAL, std::string("mptc: ") + mptc_codec_last_error());
+  DirectoryResult res;
+  try {
+    for (size_t i = 0; i < names.size(); ++i) {
+      int fw = w, fh = h;
+      if (i > 0) rgb = LoadPNG(dir + "/" + names[i], &fw, &fh);
+      if (fw != w || fh != h) throw Error(MPTC_EINVAL, "mptc: " + names[i] + " differs in size from the first frame");
+      const EncodedFrame& f = seq.Add(rgb.data(), w, h);
+      if (mptc_writer_add_frame(wr, f.physical_blocks.data(), f.motion_indices.data(), f.unique_palette.data(),
+                                (uint32_t)f.unique_palette.size()) != MPTC_OK)
+        throw Error(MPTC_EINVAL, std::string("mptc: ") + mptc_codec_last_error());
+      ++res.frames;
+    }
+    const uint8_t* data = nullptr;
+    uint64_t size = 0;
+    if (mptc_writer_finish(wr, &data, &size) != MPTC_OK)
+      throw Error(MPTC_ESTATE, std::string("mptc: ") + mptc_codec_last_error());
+    FILE* f = std::fopen(out_file.c_str(), "wb");
+    if (!f) throw Error(MPTC_EINVAL, "mptc: cannot create " + out_file);
+    const bool ok = std::fwrite(data, 1, size, f) == size;
+    if (std::fclose(f) != 0 || !ok) throw Error(MPTC_EINVAL, "mptc: cannot write " + out_file);
+    res.bytes = size;
+    res.frames_written = (res.frames / unique_interval) * unique_interval;
+  } catch (...) {
+    mptc_writer_destroy(wr);
+    throw;
+  }
+  mptc_writer_destroy(wr);
+  return res;
 }
 
 }  // namespace host

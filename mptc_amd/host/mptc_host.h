@@ -11,7 +11,14 @@
 //   SequenceEncoder  CompressMultiUnique's frame schedule: intra frames at
 //                    0, N, 2N, ...; each inter frame re-encodes against the
 //                    previous re-encoded frame (codec/codec.cpp:1379-1469,
-//                    1508); the entropy stage is not part of this path
+//                    1508)
+//   CompressDirectory  CompressMultiUnique itself (codec/codec.cpp:1307-1532):
+//                    the PNG frames of a directory in name order, encoded on
+//                    the GPU with that schedule, written as the .mptc stream
+//                    by the CPU entropy stage (include/mptc_codec.h), palette
+//                    groups of unique_interval frames
+//   LoadPNG/SavePNG  frame input as the reference's stbi_load(..., 3)
+//                    (codec/dxt_image.cpp:396) gives it, and output
 //
 // Errors: the C-ABI returns status codes; this layer turns a failed call into
 // mptc::host::Error carrying the code and mptc_last_error()'s message (the
@@ -96,6 +103,24 @@ class SequenceEncoder {
   int num_ = 1;
   int count_ = 0;
 };
+
+// RGB8 pixels of a PNG file (see png.cpp for the formats read).
+std::vector<uint8_t> LoadPNG(const std::string& path, int* width, int* height);
+void SavePNG(const std::string& path, const uint8_t* rgb, int width, int height);
+
+struct DirectoryResult {
+  int frames = 0;              // frames encoded
+  int frames_written = 0;      // frames in whole palette groups (the rest are dropped)
+  uint64_t bytes = 0;          // size of the .mptc file
+};
+
+// CompressMultiUnique (codec/codec.cpp:1307-1532): every file of `dir` in
+// name order as a PNG frame (all the same size, a block grid that is a
+// multiple of 64 in each direction), encoded with opt's search area,
+// threshold and intra interval, written to out_file with palette groups of
+// unique_interval frames.
+DirectoryResult CompressDirectory(const std::string& dir, const std::string& out_file, const Options& opt,
+                                  int unique_interval);
 
 }  // namespace host
 }  // namespace mptc

@@ -32,8 +32,7 @@ class Config(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("wg_size", C.c_int32), ("tile_w", C.c_int32), ("pass_batch", C.c_int32),
-                ("use_graph", C.c_int32), ("win_chunks", C.c_int32), ("prune", C.c_int32),
-                ("fixpoint", C.c_int32)]
+                ("use_graph", C.c_int32), ("win_chunks", C.c_int32), ("prune", C.c_int32)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -244,8 +243,8 @@ class Encoder:
         _check(self.L.mptc_get_tuning(self.h, C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, wg_size=256, tile_w=16, pass_batch=2, use_graph=0, win_chunks=4, prune=1, fixpoint=1):
-        t = Tuning(wg_size, tile_w, pass_batch, use_graph, win_chunks, prune, fixpoint)
+    def set_tuning(self, wg_size=256, tile_w=16, pass_batch=2, use_graph=0, win_chunks=4, prune=1):
+        t = Tuning(wg_size, tile_w, pass_batch, use_graph, win_chunks, prune)
         _check(self.L.mptc_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, frames: list[dict], steps: int = 3):

@@ -39,6 +39,9 @@ SEQUENCES = {
     "c3_t128": _seq(3840, 2160, 32, 3, 32, 1, 4, 10, 32, 128, golden=2),
 }
 
+# 4 x 512x256 (a 128x64 block grid: two endpoint-wavelet blocks per row), for the .mptc stream tests
+SEQUENCES["w512_t8"] = _seq(512, 256, 4, 7, 8, 1, 3, 0, 8, 8, golden=0)
+
 # 12 uneven streams encoded together: 3 each of 640x360, 1280x720, 1920x1080, 3840x2160
 STREAM_SIZES = [(640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
 for _i in range(12):

@@ -6,6 +6,7 @@ Run in the build container (where the reference sources are), e.g.
     python tests/golden/make_golden.py seq c1_s8_t0
     python tests/golden/make_golden.py all-seq      # every sequence, one process each
     python tests/golden/make_golden.py psnr         # the reference's PSNR of leading frames
+    python tests/golden/make_golden.py bitstream    # the reference's .mptc streams
 The fixtures are data only: inputs are regenerated from the seeded frame
 generator (mptc_amd/csrc/framegen.h), outputs are the reference's.
 """
@@ -160,6 +161,58 @@ def psnr_fixture() -> None:
         json.dump(out, fh, indent=1)
 
 
+# Bitstream cases: (name, sequence, frames, unique_interval).  The 512x256
+# case has a 128x64 block grid (two wavelet blocks side by side); the last
+# case leaves two frames after its last whole group.
+BITSTREAMS = [("bs_c1_s8_t0_u8", "c1_s8_t0", 8, 8), ("bs_c1_s8_t8_u4", "c1_s8_t8", 8, 4),
+              ("bs_c1_s16_t0_u3", "c1_s16_t0", 8, 3), ("bs_w512_t8_u2", "w512_t8", 4, 2)]
+
+
+def _bitstream_one(case: str) -> dict:
+    """Runs the reference's CompressMultiUnique (codec/codec.cpp:1307-1532) on
+    the PNG frames of one case in a scratch directory; returns the stream's
+    SHA-256, size and header.  One process per case: the reference keeps the
+    header's maxima in process globals."""
+    import ctypes
+    import tempfile
+    name, seq, n, ui = next(c for c in BITSTREAMS if c[0] == case)
+    spec = SEQUENCES[seq]
+    O = oracle.Oracle()
+    frames = O.synth_frames(synth_cfg(spec), n)
+    L = ctypes.CDLL(oracle.REF_CODEC_SO)
+    with tempfile.TemporaryDirectory() as tmp:
+        fdir = os.path.join(tmp, "frames")
+        os.makedirs(fdir)
+        for i, f in enumerate(frames):
+            a = np.ascontiguousarray(f)
+            rc = L.ref_write_png(os.path.join(fdir, f"frame{i:03d}.png").encode(), a.ctypes.data_as(ctypes.c_void_p),
+                                 spec["width"], spec["height"])
+            assert rc == 0
+        out = os.path.join(tmp, "out.mptc")
+        cwd = os.getcwd()
+        os.chdir(tmp)  # the reference also writes out.txt into the working directory
+        try:
+            L.ref_compress_dir(fdir.encode(), out.encode(), spec["S"], spec["thr"], spec["intra_interval"], ui)
+        finally:
+            os.chdir(cwd)
+        data = open(out, "rb").read()
+    return dict(name=name, sequence=seq, frames=n, unique_interval=ui, size=len(data), sha256=sha(np.frombuffer(data, np.uint8)),
+                header=data[:34].hex())
+
+
+def bitstream_fixture() -> None:
+    """bitstreams.json: the reference encoder's .mptc streams of the cases in
+    BITSTREAMS (SHA-256, size, header)."""
+    out = []
+    for case in BITSTREAMS:
+        r = subprocess.run([sys.executable, __file__, "bitstream-one", case[0]], capture_output=True, text=True,
+                           check=True)
+        out.append(json.loads(r.stdout.strip().splitlines()[-1]))
+        print(out[-1]["name"], out[-1]["size"], flush=True)
+    with open(os.path.join(HERE, "bitstreams.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 def main(argv):
     oracle.build_oracle()
     oracle.build_ref()
@@ -168,6 +221,15 @@ def main(argv):
         unit_vectors(oracle.Reference(), oracle.Oracle())
     elif what == "psnr":
         psnr_fixture()
+    elif what == "bitstream":
+        bitstream_fixture()
+    elif what == "bitstream-one":
+        import contextlib
+        import io
+        buf = io.StringIO()
+        with contextlib.redirect_stdout(buf):
+            r = _bitstream_one(argv[2])
+        print(json.dumps(r))
     elif what == "seq":
         for name in argv[2:]:
             sequence(name)

@@ -20,6 +20,18 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), n
 
 
+def test_codec_library_exports_every_declared_symbol():
+    import re
+
+    from mptc_amd import codec
+    L = codec.load()
+    text = open(os.path.join(os.path.dirname(lib.HEADER), "mptc_codec.h")).read()
+    names = re.findall(r"^\s*(?:int32_t|void|const char\s*\*)\s*\**\s*(mptc_\w+)\s*\(", text, re.M)
+    assert len(names) >= 9
+    for n in names:
+        assert hasattr(L, n), n
+
+
 def test_generator_matches_oracle_copy(orc):
     for name in ("c1_s8_t0", "c2_t0", "c4_s00"):
         cfg = synth_cfg(SEQUENCES[name])

@@ -80,11 +80,11 @@ def test_batch_of_uneven_streams_matches_reference(orc):
     enc.close()
 
 
-@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune,fix", [(256, 64, 2, 1, 2, 1, 0), (64, 8, 1, 0, 1, 1, 1),
-                                                             (128, 16, 2, 1, 4, 0, 2), (256, 32, 3, 0, 8, 1, 0),
-                                                             (64, 64, 16, 1, 6, 0, 1), (256, 16, 1, 1, 3, 1, 2),
-                                                             (256, 8, 2, 0, 4, 1, 1), (256, 8, 2, 0, 4, 1, 2)])
-def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks, prune, fix):
+@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune", [(256, 64, 2, 1, 2, 1), (64, 8, 1, 0, 1, 1),
+                                                         (128, 16, 2, 1, 4, 0), (256, 32, 3, 0, 8, 1),
+                                                         (64, 64, 16, 1, 6, 0), (256, 16, 1, 1, 3, 1),
+                                                         (256, 8, 2, 0, 4, 1)])
+def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks, prune):
     from mptc_amd import hip, lib
     from oracle import SynthCfg
     sizes = [(160, 96, 11), (96, 64, 12), (256, 128, 13)]
@@ -93,9 +93,9 @@ def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks, prune, fix):
     want = [orc.encode_sequence(fr, S, thr, 3) for fr in frames]
     total = sum((w // 4) * (h // 4) for w, h, _ in sizes)
     enc = lib.Encoder(256, 128, S, thr, max_batch_frames=3, max_batch_blocks=total)
-    enc.set_tuning(wg, tw, pb, graph, chunks, prune, fix)
+    enc.set_tuning(wg, tw, pb, graph, chunks, prune)
     streams = [Stream(hip, fr) for fr in frames]
-    log(f"tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune} fixpoint={fix}")
+    log(f"tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune}")
     res = _encode_streams(enc, streams, 4, 3)
     st = enc.stats()
     exact = st["exact_inter_candidates"] + st["exact_intra_candidates"]
@@ -129,9 +129,9 @@ def test_autotune_picks_a_valid_shape(orc):
     enc.close()
 
 
-@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune,fix", [(256, 8, 2, 0, 4, 1, 1), (64, 64, 1, 1, 1, 0, 0),
-                                                             (128, 32, 3, 1, 8, 1, 2), (256, 8, 2, 0, 4, 1, 0)])
-def test_tunings_at_1080p_match_reference(orc, wg, tw, pb, graph, chunks, prune, fix):
+@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune", [(256, 8, 2, 0, 4, 1), (64, 64, 1, 1, 1, 0),
+                                                         (128, 32, 3, 1, 8, 1)])
+def test_tunings_at_1080p_match_reference(orc, wg, tw, pb, graph, chunks, prune):
     """Launch shapes at the 1080p size the bench times (the first one is the
     shape the bench's autotuner picks there), against the reference's fixture."""
     from mptc_amd import hip, lib
@@ -141,9 +141,9 @@ def test_tunings_at_1080p_match_reference(orc, wg, tw, pb, graph, chunks, prune,
     n = 4
     frames = orc.synth_frames(synth_cfg(spec), n)
     enc = lib.Encoder(spec["width"], spec["height"], spec["S"], spec["thr"])
-    enc.set_tuning(wg, tw, pb, graph, chunks, prune, fix)
+    enc.set_tuning(wg, tw, pb, graph, chunks, prune)
     st = Stream(hip, frames)
-    log(f"1080p tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune} fixpoint={fix}")
+    log(f"1080p tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune}")
     res = _encode_streams(enc, [st], n, spec["intra_interval"])
     for f in range(n):
         compare_frame(res[f][0], fx["frames"][f], f"{name} frame {f}")

@@ -113,23 +113,6 @@ def test_config3_4k_s32(enc, orc, name):
     _run_sequence(enc, orc, name)
 
 
-@pytest.mark.parametrize("fix", [0, 2])
-@pytest.mark.parametrize("name", ["c3_t8", "c3_t128"])
-def test_config3_fixpoint_modes(enc, orc, name, fix):
-    """The 4K intra frame at thresholds 8 and 128 reuses nearly every block,
-    mostly in long chains of left neighbours: the host-polled passes (0) and
-    the persistent kernel with chain jumping (2) both end at the reference's
-    words."""
-    keep = enc.get_tuning()
-    enc.set_tuning(keep["wg_size"], keep["tile_w"], keep["pass_batch"], keep["use_graph"], keep["win_chunks"],
-                   keep["prune"], fix)
-    try:
-        _run_sequence(enc, orc, name)
-        log(f"{name} fixpoint {fix}: {enc.stats()['intra_iterations']} passes on the last frame")
-    finally:
-        enc.set_tuning(**keep)
-
-
 @pytest.mark.parametrize("i", range(12))
 def test_config4_streams(enc, orc, i):
     _run_sequence(enc, orc, f"c4_s{i:02d}")
@@ -187,11 +170,10 @@ def test_constant_and_two_colour_frames(enc, orc):
     _oracle_vs_gpu(enc, orc, frames, 3, 0, 8, "two-colour")
 
 
-@pytest.mark.parametrize("fix", [0, 1, 2])
-def test_gradient_chains(enc, orc, fix):
-    # smooth gradients make long chains of intra reuse (each block may copy a
-    # word a neighbour itself copied), exercising the fixpoint iterations in
-    # every fixpoint mode
+def test_gradient_chains(enc, orc):
+    # smooth gradients and flat steps make long chains of intra reuse (each
+    # block may copy a word a neighbour itself copied), exercising the
+    # fixpoint iterations
     h, w = 64, 128
     yy, xx = np.mgrid[0:h, 0:w]
     frames = []
@@ -200,14 +182,9 @@ def test_gradient_chains(enc, orc, fix):
         frames.append(img)
     frames = np.array(frames, np.uint8)
     flat = np.clip(frames.astype(int) // 16 * 16 + 3, 0, 255).astype(np.uint8)  # wide flat steps
-    keep = enc.get_tuning()
-    enc.set_tuning(**dict(keep, fixpoint=fix))
-    try:
-        for thr in (0, 2, 8, 64):
-            _oracle_vs_gpu(enc, orc, frames, 6, thr, 8, f"gradient thr={thr} fixpoint={fix}")
-            _oracle_vs_gpu(enc, orc, flat, 5, thr, 8, f"flat steps thr={thr} fixpoint={fix}")
-    finally:
-        enc.set_tuning(**keep)
+    for thr in (0, 2, 8, 64):
+        _oracle_vs_gpu(enc, orc, frames, 6, thr, 8, f"gradient thr={thr}")
+        _oracle_vs_gpu(enc, orc, flat, 5, thr, 8, f"flat steps thr={thr}")
 
 
 def test_synthetic_medium_sequences(enc, orc):
