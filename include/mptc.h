@@ -265,6 +265,26 @@ int32_t mptc_get_proposals(mptc_ctx* ctx, int32_t net, int16_t* out, int64_t n_b
  * proposed for every block. */
 int32_t mptc_set_prior(mptc_ctx* ctx, int32_t net, const float* scores);
 
+/* Allocation-free variant of mptc_encode_frame_device (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code inside
+ * a serving loop that must not call hipMalloc, harnesses that forbid it.  It
+ * takes no context: the work buffers live in a device array inside this
+ * library (one copy per device, about 53 MiB, held until the process ends),
+ * and the call makes no device or host allocation after its first call on a
+ * device (which looks up that array's address).  Limits: frames of at most
+ * 65536 4x4 blocks (1280x720 = 57,600 fits, 1920x1080 does not) and
+ * search_area 1..8; larger inputs return MPTC_EINVAL.  Same inputs, outputs
+ * (device pointers) and bytes as mptc_encode_frame_device with that search
+ * area and threshold.  Work is queued on `hip_stream` (NULL = the null
+ * stream) and the call returns once the frame is done.  Calls on one device
+ * share its array: they are serialised within the process (a call waits for
+ * the previous one), and no state carries over from one call to the next. */
+int32_t mptc_encode_frame_noalloc(int32_t device, const uint8_t* d_rgb, int32_t width, int32_t height,
+                                  int32_t search_area, int32_t err_threshold, int32_t is_intra,
+                                  const uint64_t* d_ref_phys, uint64_t* d_phys_out, uint8_t* d_motion_out,
+                                  uint32_t* d_unique_out, uint32_t* d_n_unique_out, uint8_t* d_eps_out,
+                                  uint8_t* d_kind_out, void* hip_stream);
+
 /* Seeded synthetic RGB8 frames (see mptc_amd/csrc/framegen.h): writes
  * n_frames frames starting at first_frame, each width*height*3 bytes.
  * The layout of cfg is mptc_synth_cfg. */

@@ -1498,3 +1498,198 @@ int32_t mptc_synth_frames(const void* cfg, int32_t first, int32_t n, uint8_t* rg
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Entry point that allocates nothing: mptc_encode_frame_noalloc.
+//
+// The work buffers of one frame of up to kNaMaxBlocks blocks at search areas
+// up to kNaMaxS live in `scratch`, a device global: every device this library
+// runs on holds its own copy of it (about 53 MiB) until the process ends.  On
+// a device's first call the entry point looks up that copy's address
+// (hipGetSymbolAddress) and lays the buffers out in it; host code never uses
+// the array's name as an address.  Calls on one device are serialised (one
+// mutex per device) and each returns once its frame is done, so calls never
+// overlap on the array; each call's first kernel clears the counters the
+// frame needs (nothing depends on what the previous call left).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNaMaxS = 8;
+constexpr int kNaMaxBlocks = 65536;  // e.g. 1280x720 (57,600 blocks) and every smaller frame
+constexpr size_t kNaNbs = kNaMaxBlocks;  // E row stride, a multiple of 64
+
+enum NaBuf {
+  NA_FR, NA_SID, NA_TSID, NA_NEED, NA_OWN, NA_INTER, NA_INTRA, NA_RW, NA_CW, NA_CWN, NA_DIRTY, NA_TILES,
+  NA_RTAB, NA_CTAB, NA_ORIG, NA_CHG, NA_E, NA_SUMM, NA_STATS, NA_RBT, NA_CBT, NA_PM, NA_CONV, NA_N
+};
+
+constexpr size_t na_align(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+// Bytes of each work buffer (the sizes mptc_create allocates for one frame
+// of kNaMaxBlocks blocks at S = kNaMaxS).
+constexpr size_t na_bytes(int b) {
+  switch (b) {
+    case NA_FR: return sizeof(FrameDev);
+    case NA_SID: case NA_TSID: case NA_NEED: return kNaNbs;  // a wave tile holds at least one block
+    case NA_OWN: case NA_INTER: case NA_INTRA: return 8 * kNaNbs;
+    case NA_RW: case NA_CW: case NA_CWN: case NA_DIRTY: case NA_ORIG: case NA_CHG: return 4 * kNaNbs;
+    case NA_TILES: return 4 * (kNaMaxBlocks / kUniqueTile + 2);
+    case NA_RTAB: case NA_CTAB: return sizeof(float4) * kNaNbs;
+    case NA_E: return static_cast<size_t>(kNaMaxS) * kNaMaxS * kNaNbs * sizeof(uint2);
+    case NA_SUMM: return static_cast<size_t>(kMaxChunks) * kNaNbs * sizeof(uint4);
+    case NA_STATS: return sizeof(DevStats);
+    case NA_RBT: case NA_CBT: return sizeof(BoundTab) * kNaNbs;
+    case NA_PM: return sizeof(uint32_t) * kPassWords * kNaNbs;
+    case NA_CONV: return 64 * sizeof(uint32_t);
+    default: return 0;
+  }
+}
+
+constexpr size_t na_offset(int b) {
+  size_t off = 0;
+  for (int i = 0; i < b; ++i) off = na_align(off + na_bytes(i));
+  return off;
+}
+
+constexpr size_t kNaScratchBytes = na_offset(NA_N);
+static_assert(kNaScratchBytes <= (size_t)56 << 20, "scratch stays below 56 MiB");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+namespace {
+
+__global__ void __launch_bounds__(256) k_zero_bytes(uint8_t* __restrict__ p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+struct NoAllocDev {
+  std::mutex mu;           // calls on one device run one at a time
+  bool ready = false;      // buffers laid out in this device's copy of `scratch`
+  mptc_ctx ctx;            // work-buffer pointers into `scratch`; no allocation of its own
+  FrameDev h_fr{};         // staging of the frame descriptor
+  uint32_t h_conv[64] = {};  // host side of the context's status words (the kernels write the device side)
+  uint32_t h_changed = 0;    // changed words of the last fixpoint pass, copied back
+};
+
+NoAllocDev g_na[kMaxDevices];
+
+// Once per device, with the device current: the address of this device's copy
+// of `scratch` and the buffer layout in it.
+int na_setup(int device, NoAllocDev& nd) {
+  int rc = init_tables(device);
+  if (rc) return rc;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(MPTC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  unsigned char* base = static_cast<unsigned char*>(p);
+  auto at = [&](int b) { return static_cast<void*>(base + na_offset(b)); };
+  mptc_ctx& c = nd.ctx;
+  c.cfg = mptc_config{device, 4, 4, kNaMaxS, 0, 1, kNaMaxBlocks};
+  c.alloc_S = kNaMaxS;
+  c.max_total = kNaMaxBlocks;
+  c.max_frames = 1;
+  c.nbs = kNaNbs;
+  c.d_fr = static_cast<FrameDev*>(at(NA_FR));
+  c.d_sid = static_cast<uint8_t*>(at(NA_SID));
+  c.d_tsid = static_cast<uint8_t*>(at(NA_TSID));
+  c.d_need = static_cast<uint8_t*>(at(NA_NEED));
+  c.d_own = static_cast<uint64_t*>(at(NA_OWN));
+  c.d_inter = static_cast<uint64_t*>(at(NA_INTER));
+  c.d_intra = static_cast<uint64_t*>(at(NA_INTRA));
+  c.d_rw = static_cast<uint32_t*>(at(NA_RW));
+  c.d_cw = static_cast<uint32_t*>(at(NA_CW));
+  c.d_cwn = static_cast<uint32_t*>(at(NA_CWN));
+  c.d_dirty = static_cast<uint32_t*>(at(NA_DIRTY));
+  c.d_tiles = static_cast<uint32_t*>(at(NA_TILES));
+  c.d_rtab = static_cast<float4*>(at(NA_RTAB));
+  c.d_ctab = static_cast<float4*>(at(NA_CTAB));
+  c.d_orig = static_cast<int*>(at(NA_ORIG));
+  c.d_chg = static_cast<int*>(at(NA_CHG));
+  c.d_E = static_cast<uint2*>(at(NA_E));
+  c.d_summ = static_cast<uint4*>(at(NA_SUMM));
+  c.d_stats = static_cast<DevStats*>(at(NA_STATS));
+  c.d_rbt = static_cast<BoundTab*>(at(NA_RBT));
+  c.d_cbt = static_cast<BoundTab*>(at(NA_CBT));
+  c.d_pm = static_cast<uint32_t*>(at(NA_PM));
+  c.d_conv = static_cast<uint32_t*>(at(NA_CONV));
+  c.h_conv = nd.h_conv;
+  c.tune = mptc_tuning{256, 16, 2, 0, 4, 1};  // direct launches
+  int cu = 0;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c.n_cu = cu;
+  nd.ready = true;
+  return MPTC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mptc_encode_frame_noalloc(int32_t device, const uint8_t* d_rgb, int32_t w, int32_t h, int32_t search_area,
+                                  int32_t err_threshold, int32_t is_intra, const uint64_t* d_ref_phys,
+                                  uint64_t* d_phys_out, uint8_t* d_motion_out, uint32_t* d_unique_out,
+                                  uint32_t* d_n_unique_out, uint8_t* d_eps_out, uint8_t* d_kind_out,
+                                  void* hip_stream) {
+  if (device < 0 || device >= kMaxDevices) return fail(MPTC_EINVAL, "no such HIP device");
+  if (w <= 0 || h <= 0 || (w & 3) || (h & 3))
+    return fail(MPTC_EINVAL, "width and height must be positive multiples of 4");
+  if ((long long)(w / 4) * (h / 4) > kNaMaxBlocks)
+    return fail(MPTC_EINVAL, "frame larger than the allocation-free maximum of 65536 blocks");
+  if (search_area < 1 || search_area > kNaMaxS)
+    return fail(MPTC_EINVAL, "search_area must be in 1..8 for the allocation-free entry point");
+  int rc = check_params(search_area, err_threshold);
+  if (rc) return rc;
+  if (!d_rgb || !d_phys_out || !d_motion_out || !d_unique_out || !d_n_unique_out)
+    return fail(MPTC_EINVAL, "null argument");
+  if (!is_intra && !d_ref_phys) return fail(MPTC_EINVAL, "inter frame needs the previous frame's blocks");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device >= ndev) return fail(MPTC_EINVAL, "no such HIP device");
+  NoAllocDev& nd = g_na[device];
+  std::lock_guard<std::mutex> lk(nd.mu);
+  HIP_TRY(hipSetDevice(device));
+  if (!nd.ready && (rc = na_setup(device, nd))) return rc;
+  mptc_ctx* c = &nd.ctx;
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  c->cfg.search_area = search_area;
+  c->cfg.err_threshold = err_threshold;
+  const FrameIn in{d_rgb,          w, h, is_intra, is_intra ? nullptr : d_ref_phys, d_phys_out, d_motion_out,
+                   d_unique_out, d_n_unique_out, d_eps_out, d_kind_out};
+  int base = 0, tb = 0, wt = 0, lt = 0;
+  nd.h_fr = frame_dev(in, c->tune.tile_w, base, tb, wt, lt);
+  HIP_TRY(hipMemcpyAsync(c->d_fr, &nd.h_fr, sizeof(FrameDev), hipMemcpyHostToDevice, s));
+  // one frame: every block and wave tile belongs to frame 0; the pass count
+  // word cleared (a step whose first search changes nothing has no pass that
+  // writes it)
+  hipLaunchKernelGGL(k_fill_sid, dim3(grid_for(base, 256), 1), dim3(256), 0, s, c->d_fr, 1, c->d_sid);
+  hipLaunchKernelGGL(k_zero_bytes, dim3(grid_for(wt, 256)), dim3(256), 0, s, c->d_tsid, wt);
+  hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(256), 0, s, reinterpret_cast<uint8_t*>(c->d_conv),
+                     (int)(64 * sizeof(uint32_t)));
+  HIP_TRY(hipGetLastError());
+  c->n_frames = 1;
+  c->total = base;
+  c->n_wtiles = wt;
+  c->n_tiles = tb;
+  c->n_ltiles = lt;
+  c->any_inter = !is_intra;
+  c->last_intra = is_intra;
+  if ((rc = queue_prefix(c, s, true))) return rc;
+  // fixpoint passes in batches; after each batch the count of changed words
+  // of its last pass comes back by a plain copy (no event, no mapped memory)
+  const int np = passes_per_batch(c);
+  for (int passes = 0;; passes += np) {
+    if (passes > c->total + 4 * np) return fail(MPTC_ESTATE, "intra fixpoint did not converge");
+    if ((rc = queue_passes(c, s))) return rc;
+    HIP_TRY(hipMemcpyAsync(&nd.h_changed, c->d_conv, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (nd.h_changed == 0) break;
+  }
+  if ((rc = queue_suffix(c, s))) return rc;
+  HIP_TRY(hipStreamSynchronize(s));  // the frame is done when the call returns
+  return MPTC_OK;
+}
+
+}  // extern "C"
+

@@ -119,6 +119,8 @@ def load() -> C.CDLL:
     L.mptc_set_search_mode.argtypes = [_vp, C.c_int32, C.c_int32]
     L.mptc_set_prior.argtypes = [_vp, C.c_int32, _vp]
     L.mptc_get_proposals.argtypes = [_vp, C.c_int32, _vp, C.c_int64]
+    L.mptc_encode_frame_noalloc.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                            C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     for name in header_functions():
         if name not in ("mptc_destroy", "mptc_last_error"):
             getattr(L, name).restype = C.c_int32
@@ -337,6 +339,16 @@ class Encoder:
             prev = r["phys"]
             set_intra, num = next_intra(set_intra, num, intra_interval)
         return out
+
+
+def encode_frame_noalloc(device: int, d_rgb, w: int, h: int, search_area: int, err_threshold: int,
+                         is_intra: bool, d_ref, d_phys, d_motion, d_unique, d_n_unique, d_eps=None, d_kind=None,
+                         stream: int | None = None):
+    """mptc_encode_frame_noalloc: one frame, no context, no allocation;
+    returns once the frame is done."""
+    _check(load().mptc_encode_frame_noalloc(device, _ptr(d_rgb), w, h, search_area, err_threshold,
+                                            int(bool(is_intra)), _ptr(d_ref), _ptr(d_phys), _ptr(d_motion),
+                                            _ptr(d_unique), _ptr(d_n_unique), _ptr(d_eps), _ptr(d_kind), stream))
 
 
 def next_intra(set_intra: bool, num: int, interval: int):
