@@ -316,13 +316,17 @@ class Runner:
 
     def quality(self, steps):
         """Mean decoder-side PSNR of the frames the last run wrote (the last
-        min(steps, n_out) of them), and their unique-block count."""
-        import oracle
-        O = oracle.Oracle()
+        min(steps, n_out) of them), from mptc_frame_sse on the device."""
+        lib = self.lib
+        fb = self.W * self.H * 3
+        d_sse = self.hip.Buffer(8)
         ps = []
         for f in range(max(0, steps - self.n_out), steps):
-            phys = self.d_phys[f % self.n_out].download(np.uint64, self.nb)
-            ps.append(O.psnr_physical(self.frames[f % self.n_seq], phys))
+            lib.frame_sse(self.d_frames.offset((f % self.n_seq) * fb), self.W, self.H, self.d_phys[f % self.n_out],
+                          d_sse, self.stream.handle)
+            self.stream.synchronize()
+            ps.append(lib.psnr_from_sse(int(d_sse.download(np.uint64, 1)[0]), self.W, self.H))
+        d_sse.free()
         return float(np.mean(ps))
 
 

@@ -61,6 +61,8 @@ typedef struct {
   int32_t prune;      /* 1: for S <= 8 and thresholds 0..4, evaluate exactly only the
                          candidates an error lower bound cannot rule out (same output);
                          0: evaluate every candidate */
+  int32_t ltile_w;    /* learned intra search: a wave settles reuse chains inside a tile of
+                         ltile_w x 64/ltile_w blocks per pass; 8, 16, 32 or 64 */
 } mptc_tuning;
 
 /* One frame of a batched step: device pointers, layouts as mptc_reencode. */
@@ -145,7 +147,8 @@ int32_t mptc_encode_batch_device(mptc_ctx* ctx, int32_t n_frames, const mptc_fra
                                  void* hip_stream);
 
 /* Launch-shape tuning.  mptc_autotune times every candidate shape (wg_size x
- * tile_w x win_chunks x use_graph, and prune 0/1 where the pruned search applies) on
+ * tile_w x win_chunks x use_graph, prune 0/1 where the pruned search applies,
+ * ltile_w in the learned intra search) on
  * `steps` encodes of the given batch (outputs are overwritten), keeps the
  * fastest and reports it. */
 int32_t mptc_get_tuning(mptc_ctx* ctx, mptc_tuning* out);
@@ -264,6 +267,12 @@ int32_t mptc_get_proposals(mptc_ctx* ctx, int32_t net, int16_t* out, int64_t n_b
  * MPTC_NET_LOGITS floats; NULL returns to the network): the same offsets are
  * proposed for every block. */
 int32_t mptc_set_prior(mptc_ctx* ctx, int32_t net, const float* scores);
+/* Fixed base scores under a network's logits (host, MPTC_NET_LOGITS floats;
+ * NULL removes them): logit L = network output L + base[L] wherever the
+ * network is used -- training loss, mptc_net_forward and the learned search.
+ * With the logit columns of W3 and b3 at zero the network then starts as the
+ * fixed ranking the base gives and learns corrections to it. */
+int32_t mptc_net_set_base(mptc_ctx* ctx, int32_t net, const float* scores);
 
 /* Allocation-free variant of mptc_encode_frame_device (no counterpart in the
  * reference): for callers that cannot allocate device memory -- code inside
@@ -284,6 +293,15 @@ int32_t mptc_encode_frame_noalloc(int32_t device, const uint8_t* d_rgb, int32_t 
                                   const uint64_t* d_ref_phys, uint64_t* d_phys_out, uint8_t* d_motion_out,
                                   uint32_t* d_unique_out, uint32_t* d_n_unique_out, uint8_t* d_eps_out,
                                   uint8_t* d_kind_out, void* hip_stream);
+
+/* Quality of an encoded frame: *d_sse_out (device, one uint64) receives the
+ * sum over all pixels and channels of the squared difference between the
+ * RGB8 frame and the decoding of its physical blocks d_phys (device, raster
+ * order).  PSNR = 10 log10(3 * 255^2 * width * height / sse), the formula of
+ * DXTImage::PSNR (codec/dxt_image.cpp:363-383) over the blocks a decoder
+ * reads.  Queued on hip_stream (NULL: the null stream); no context needed. */
+int32_t mptc_frame_sse(const uint8_t* d_rgb, int32_t width, int32_t height, const uint64_t* d_phys,
+                       uint64_t* d_sse_out, void* hip_stream);
 
 /* Seeded synthetic RGB8 frames (see mptc_amd/csrc/framegen.h): writes
  * n_frames frames starting at first_frame, each width*height*3 bytes.

@@ -74,6 +74,7 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
 // ---------------------------------------------------------------------------
 struct GemmEpi {
   const float* bias;     // [N] or null
+  const float* bias2;    // [N] or null: a second, constant bias added after `bias`
   int relu;              // max(0, .) before storing
   const uint16_t* mask;  // bf16 [M x N] (row stride ldm): result zeroed where mask <= 0; or null
   long ldm;
@@ -162,6 +163,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const ui
         if (m >= M || n >= N) continue;
         float v = acc[i][j][t];
         if (ep.bias) v += ep.bias[n];
+        if (ep.bias2) v += ep.bias2[n];
         if (ep.relu) v = fmaxf(v, 0.0f);
         if (ep.mask && !(bf2f(ep.mask[(long)m * ep.ldm + n]) > 0.0f)) v = 0.0f;
         if (Cf) Cf[ep.trans_f ? (long)n * ep.ldc + m : (long)m * ep.ldc + n] = v;
@@ -427,8 +429,10 @@ __global__ void __launch_bounds__(256) k_block_prep(const FrameDev* __restrict__
   if (X) net_input_row(px, X + (size_t)g * kNetIn);
 }
 
-// Learned intra search, one fixpoint pass.  A wave owns an 8x8 tile of
-// blocks (lane = 8 * row + column in the tile).  Every block that needs the
+// Learned intra search, one fixpoint pass.  A wave owns a tile of
+// 2^ltw_log x 64 / 2^ltw_log blocks (lane = row * width + column in the tile;
+// wider tiles settle longer horizontal chains within a pass, taller ones
+// vertical chains).  Every block that needs the
 // intra search evaluates its proposed offsets against the current words --
 // words of its own tile come from the lanes that own them, others from
 // memory -- and once all its evaluations are current it takes the valid
@@ -451,7 +455,7 @@ constexpr int kNotEvaluated = (int)0x80000000;
 
 template <int K>
 __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict__ fr, int n_frames, int n_ltiles,
-                                                     int S, int thr, int kk, const int16_t* __restrict__ prop,
+                                                     int ltw_log, int S, int thr, int kk, const int16_t* __restrict__ prop,
                                                      const uint64_t* __restrict__ own, const uint8_t* __restrict__ need,
                                                      uint32_t* __restrict__ cw, uint32_t* __restrict__ seen,
                                                      int* __restrict__ errs, uint64_t* __restrict__ dec,
@@ -480,8 +484,10 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
     int f = 0;
     while (f + 1 < n_frames && fr[f + 1].ltile_base <= wv) ++f;
     const FrameDev& d = fr[f];
+    // the wave's tile: ltw x lth blocks, lane = row * ltw + column
+    const int ltw = 1 << ltw_log, lth_log = 6 - ltw_log;
     const int lt = wv - d.ltile_base, tx = lt % d.ltiles_x, ty = lt / d.ltiles_x;
-    const int bx = tx * 8 + (lane & 7), by = ty * 8 + (lane >> 3);
+    const int bx = (tx << ltw_log) + (lane & (ltw - 1)), by = (ty << lth_log) + (lane >> ltw_log);
     const bool inside = bx < d.bw && by < d.bh;
     const int g = inside ? d.base + by * d.bw + bx : 0;
     const bool act = inside && need[g];
@@ -501,8 +507,8 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
       er[i] = kNotEvaluated;
       if (L >= 0) {
         const int qx = bx + L % kNetWin - 8, qy = by - L / kNetWin;
-        if ((qx >> 3) == tx && (qy >> 3) == ty)
-          src[i] = (qy & 7) * 8 + (qx & 7);
+        if ((qx >> ltw_log) == tx && (qy >> lth_log) == ty)
+          src[i] = ((qy & ((1 << lth_log) - 1)) << ltw_log) | (qx & (ltw - 1));
         else
           w[i] = cw[d.base + qy * d.bw + qx];
         if (!first) {

@@ -148,6 +148,8 @@ struct NetState {
   bool ready = false;      // parameters set
   float* prior = nullptr;  // kNetLogits fixed scores
   bool use_prior = false;
+  float* base = nullptr;   // kNetOut fixed scores added to the logits (seed outputs 0)
+  bool use_base = false;
   uint16_t* pk = nullptr;  // packed inference image (net_infer.h)
 };
 
@@ -180,7 +182,7 @@ struct mptc_ctx {
   // work on the context's buffers is ordered whatever streams callers use.
   hipEvent_t done_ev = nullptr;
   hipStream_t done_stream = nullptr;  // the stream done_ev was last recorded on (null: never)
-  mptc_tuning tune{256, 16, 2, 0, 4, 1};  // the fastest 1080p shape measured on MI355X
+  mptc_tuning tune{256, 16, 2, 0, 4, 1, 8};  // the fastest 1080p shape measured on MI355X
   // device buffers (one entry per block of the batch unless noted)
   FrameDev* d_fr = nullptr;  // kMaxFrames descriptors
   FrameDev* h_fr = nullptr;  // pinned staging ring: 4 x kMaxFrames
@@ -204,7 +206,7 @@ struct mptc_ctx {
   uint32_t *d_unique = nullptr, *d_nunique = nullptr;
   // batch shape the maps and graphs were built for
   std::vector<int> shape;
-  mptc_tuning shape_tune{0, 0, 0, 0, 0, 0};
+  mptc_tuning shape_tune{0, 0, 0, 0, 0, 0, 0};
   int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0, n_ltiles = 0;
   GraphSet graphs;
   bool any_inter = true;  // the batch holds an inter frame
@@ -302,6 +304,8 @@ int check_tuning(const mptc_tuning& t) {
   if (t.pass_batch < 1 || t.pass_batch > 16) return fail(MPTC_EINVAL, "pass_batch must be in 1..16");
   if (t.win_chunks < 1 || t.win_chunks > kMaxChunks) return fail(MPTC_EINVAL, "win_chunks must be in 1..8");
   if (t.prune != 0 && t.prune != 1) return fail(MPTC_EINVAL, "prune must be 0 or 1");
+  if (t.ltile_w != 8 && t.ltile_w != 16 && t.ltile_w != 32 && t.ltile_w != 64)
+    return fail(MPTC_EINVAL, "ltile_w must be 8, 16, 32 or 64");
   return MPTC_OK;
 }
 
@@ -331,13 +335,14 @@ int net_alloc(NetState& n) {
   int rc;
   if ((rc = dalloc(&n.w, kNetParams)) || (rc = dalloc(&n.wb, kNetParams)) || (rc = dalloc(&n.m, kNetParams)) ||
       (rc = dalloc(&n.v, kNetParams)) || (rc = dalloc(&n.g, kNetParams)) || (rc = dalloc(&n.prior, kNetLogits)) ||
-      (rc = dalloc(&n.pk, kPkSize)))
+      (rc = dalloc(&n.pk, kPkSize)) || (rc = dalloc(&n.base, kNetOut)))
     return rc;
+  if (hipMemset(n.base, 0, sizeof(float) * kNetOut) != hipSuccess) return fail(MPTC_EHIP, "hipMemset failed");
   return MPTC_OK;
 }
 
 void net_free(NetState& n) {
-  void* b[] = {n.w, n.wb, n.m, n.v, n.g, n.prior, n.pk};
+  void* b[] = {n.w, n.wb, n.m, n.v, n.g, n.prior, n.pk, n.base};
   for (void* p : b)
     if (p) (void)hipFree(p);
   n = NetState();
@@ -435,6 +440,7 @@ int net_forward_q(mptc_ctx* c, hipStream_t s, const NetState& n, NetWork& w, int
     return rc;
   e = epi();
   e.bias = n.w + kOffB3;
+  e.bias2 = n.use_base ? n.base : nullptr;
   e.Cf = y;
   e.trans_f = trans ? 1 : 0;
   e.ldc = trans ? rows : kNetOut;
@@ -514,9 +520,9 @@ struct FrameIn {
 
 // The device descriptor of one frame of a batch whose earlier frames hold
 // `base` blocks, `tb` unique tiles, `wt` wave tiles of width tw and `lt`
-// learned-pass tiles; advances the four counts past this frame.
-FrameDev frame_dev(const FrameIn& x, int tw, int& base, int& tb, int& wt, int& lt) {
-  const int th = 64 / tw;
+// learned-pass tiles of width ltw; advances the four counts past this frame.
+FrameDev frame_dev(const FrameIn& x, int tw, int ltw, int& base, int& tb, int& wt, int& lt) {
+  const int th = 64 / tw, lth = 64 / ltw;
   FrameDev d{};
   d.rgb = x.rgb; d.ref = x.ref; d.phys_out = x.phys; d.motion = x.motion; d.unique = x.unique;
   d.n_unique = x.n_unique; d.eps = x.eps; d.kind = x.kind;
@@ -526,8 +532,8 @@ FrameDev frame_dev(const FrameIn& x, int tw, int& base, int& tb, int& wt, int& l
   d.tiles_x = (d.bw + tw - 1) / tw;
   d.wtile_base = wt;
   d.ltile_base = lt;
-  d.ltiles_x = (d.bw + 7) / 8;
-  lt += d.ltiles_x * ((d.bh + 7) / 8);
+  d.ltiles_x = (d.bw + ltw - 1) / ltw;
+  lt += d.ltiles_x * ((d.bh + lth - 1) / lth);
   base += d.nb;
   tb += d.n_tiles;
   wt += d.tiles_x * ((d.bh + th - 1) / th);
@@ -548,7 +554,7 @@ int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
       return fail(MPTC_EINVAL, "width and height must be positive multiples of 4");
     if (!x.rgb || !x.phys || !x.motion || !x.unique || !x.n_unique) return fail(MPTC_EINVAL, "null argument");
     if (!x.is_intra && !x.ref) return fail(MPTC_EINVAL, "inter frame needs the previous frame's blocks");
-    const FrameDev d = frame_dev(x, tw, base, tb, wt, lt);
+    const FrameDev d = frame_dev(x, tw, c->tune.ltile_w, base, tb, wt, lt);
     if (base > c->max_total) return fail(MPTC_EINVAL, "batch holds more blocks than the context allows");
     h[f] = d;
     shape.push_back(x.w);
@@ -652,7 +658,7 @@ int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first) {
   return launch(c, s, KF_LPASS, [&] {
     auto kern = kk <= 4 ? k_lintra_pass<4> : kk <= 8 ? k_lintra_pass<8> : k_lintra_pass<16>;
     hipLaunchKernelGGL(kern, dim3(grid_for(c->n_ltiles * 64, 256)), dim3(256), 0, s, c->d_fr, c->n_frames,
-                       c->n_ltiles, S, thr, kk, c->d_prop, c->d_own, c->d_need, c->d_cw, c->d_seen, c->d_errs,
+                       c->n_ltiles, __builtin_ctz((unsigned)c->tune.ltile_w), S, thr, kk, c->d_prop, c->d_own, c->d_need, c->d_cw, c->d_seen, c->d_errs,
                        c->d_intra, c->d_stats, par, first, c->d_conv);
   });
 }
@@ -1143,12 +1149,16 @@ int32_t mptc_autotune(mptc_ctx* c, int32_t n, const mptc_frame_io* frames, int32
   probe.prune = 1;
   c->tune = probe;
   const int n_prune = use_prune(c) ? 2 : 1;
+  const int ltws[] = {8, 16, 32, 64};
+  const int n_lt = c->mode == MPTC_SEARCH_LEARNED ? 4 : 1;  // the learned intra passes' tile
+  for (int lti = 0; lti < n_lt; ++lti)
   for (int pr = 0; pr < n_prune; ++pr)
   for (int graph = 0; graph < 2; ++graph)
   for (int wg : wgs)
     for (int tw : tws)
       for (int kc : chunks) {
-        c->tune = mptc_tuning{wg, tw, keep.pass_batch, graph, kc, n_prune == 2 ? pr : keep.prune};
+        c->tune = mptc_tuning{wg, tw, keep.pass_batch, graph, kc, n_prune == 2 ? pr : keep.prune,
+                              n_lt > 1 ? ltws[lti] : keep.ltile_w};
         int rc = mptc_encode_batch_device(c, n, frames, nullptr);  // warm-up: maps and graphs
         if (rc) {
           c->tune = keep;
@@ -1255,7 +1265,8 @@ static int upload_params(mptc_ctx* c, NetState& n, const float* params) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemsetAsync(n.m, 0, sizeof(float) * kNetParams, s));
   HIP_TRY(hipMemsetAsync(n.v, 0, sizeof(float) * kNetParams, s));
-  hipLaunchKernelGGL(k_pack_infer, dim3(grid_for(kL3Rows * kNetHid, 256)), dim3(256), 0, s, n.w, n.pk);
+  hipLaunchKernelGGL(k_pack_infer, dim3(grid_for(kL3Rows * kNetHid, 256)), dim3(256), 0, s, n.w,
+                     n.use_base ? n.base : nullptr, n.pk);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s));
   n.t = 0;
@@ -1400,7 +1411,8 @@ int32_t mptc_net_train(mptc_ctx* c, int32_t net, const uint8_t* d_px, const uint
                        kNetParams, lr, b1, b2, c1, c2, epsv);
     if (hipGetLastError() != hipSuccess) return done(fail(MPTC_EHIP, "k_adam launch failed"));
   }
-  hipLaunchKernelGGL(k_pack_infer, dim3(grid_for(kL3Rows * kNetHid, 256)), dim3(256), 0, s, ns.w, ns.pk);
+  hipLaunchKernelGGL(k_pack_infer, dim3(grid_for(kL3Rows * kNetHid, 256)), dim3(256), 0, s, ns.w,
+                     ns.use_base ? ns.base : nullptr, ns.pk);
   if (hipGetLastError() != hipSuccess) return done(fail(MPTC_EHIP, "k_pack_infer launch failed"));
   if (loss_out &&
       hipMemcpyAsync(loss_out, d_losses, sizeof(float) * cfg->steps, hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -1472,6 +1484,24 @@ int32_t mptc_get_proposals(mptc_ctx* c, int32_t net, int16_t* out, int64_t n_blo
   return MPTC_OK;
 }
 
+int32_t mptc_net_set_base(mptc_ctx* c, int32_t net, const float* scores) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  NetState& n = c->net[net];
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemsetAsync(n.base, 0, sizeof(float) * kNetOut, s));
+  if (scores) HIP_TRY(hipMemcpyAsync(n.base, scores, sizeof(float) * kNetLogits, hipMemcpyHostToDevice, s));
+  n.use_base = scores != nullptr;
+  if (n.ready) {  // the inference image carries the base in its layer-3 biases
+    hipLaunchKernelGGL(k_pack_infer, dim3(grid_for(kL3Rows * kNetHid, 256)), dim3(256), 0, s, n.w,
+                       n.use_base ? n.base : nullptr, n.pk);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  c->graphs.destroy();
+  return MPTC_OK;
+}
+
 int32_t mptc_set_prior(mptc_ctx* c, int32_t net, const float* scores) {
   int rc = check_net(c, net);
   if (rc) return rc;
@@ -1485,6 +1515,50 @@ int32_t mptc_set_prior(mptc_ctx* c, int32_t net, const float* scores) {
     n.use_prior = false;
   }
   c->graphs.destroy();
+  return MPTC_OK;
+}
+
+}  // extern "C"
+
+// Sum of squared RGB differences between a frame and the decoding of its
+// physical blocks (what a decoder of the stream shows): the numerator of
+// DXTImage::PSNR (codec/dxt_image.cpp:363-383), taken over physical blocks.
+// One lane per block, a wave sum, one 64-bit atomic per wave.
+__global__ void __launch_bounds__(256) k_frame_sse(const uint8_t* __restrict__ rgb, int w, int nb,
+                                                   const uint64_t* __restrict__ phys,
+                                                   unsigned long long* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long sse = 0;
+  if (k < nb) {
+    const int bw = w >> 2;
+    BlockPx px;
+    load_block(rgb, w, k % bw, k / bw, px);
+    const uint64_t p = phys[k];
+    uint32_t pal[4];
+    phys_palette(p, pal);
+    sse = block_sse(px, (uint32_t)(p >> 32), pal);
+  }
+  for (int o = 32; o > 0; o >>= 1) sse += __shfl_down(sse, o);
+  if ((threadIdx.x & 63) == 0 && sse) atomicAdd(out, sse);
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0; }
+
+extern "C" {
+
+int32_t mptc_frame_sse(const uint8_t* d_rgb, int32_t w, int32_t h, const uint64_t* d_phys, uint64_t* d_sse_out,
+                       void* hip_stream) {
+  if (!d_rgb || !d_phys || !d_sse_out) return fail(MPTC_EINVAL, "null argument");
+  if (w <= 0 || h <= 0 || (w & 3) || (h & 3)) return fail(MPTC_EINVAL, "width and height must be positive multiples of 4");
+  const long nbl = (long)(w >> 2) * (h >> 2);
+  if (nbl > (1L << 30)) return fail(MPTC_EINVAL, "frame too large");
+  const int nb = (int)nbl;
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  auto* out = reinterpret_cast<unsigned long long*>(d_sse_out);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, out);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_frame_sse, dim3(grid_for(nb, 256)), dim3(256), 0, s, d_rgb, w, nb, d_phys, out);
+  HIP_TRY(hipGetLastError());
   return MPTC_OK;
 }
 
@@ -1616,7 +1690,7 @@ int na_setup(int device, NoAllocDev& nd) {
   c.d_pm = static_cast<uint32_t*>(at(NA_PM));
   c.d_conv = static_cast<uint32_t*>(at(NA_CONV));
   c.h_conv = nd.h_conv;
-  c.tune = mptc_tuning{256, 16, 2, 0, 4, 1};  // direct launches
+  c.tune = mptc_tuning{256, 16, 2, 0, 4, 1, 8};  // direct launches
   int cu = 0;
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c.n_cu = cu;
   nd.ready = true;
@@ -1658,7 +1732,7 @@ int32_t mptc_encode_frame_noalloc(int32_t device, const uint8_t* d_rgb, int32_t 
   const FrameIn in{d_rgb,          w, h, is_intra, is_intra ? nullptr : d_ref_phys, d_phys_out, d_motion_out,
                    d_unique_out, d_n_unique_out, d_eps_out, d_kind_out};
   int base = 0, tb = 0, wt = 0, lt = 0;
-  nd.h_fr = frame_dev(in, c->tune.tile_w, base, tb, wt, lt);
+  nd.h_fr = frame_dev(in, c->tune.tile_w, c->tune.ltile_w, base, tb, wt, lt);
   HIP_TRY(hipMemcpyAsync(c->d_fr, &nd.h_fr, sizeof(FrameDev), hipMemcpyHostToDevice, s));
   // one frame: every block and wave tile belongs to frame 0; the pass count
   // word cleared (a step whose first search changes nothing has no pass that

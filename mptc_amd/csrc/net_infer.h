@@ -46,8 +46,10 @@ __host__ __device__ __forceinline__ int perm_unit(int p) {
   return 16 * s + 8 * (j >> 2) + 4 * h + (j & 3);
 }
 
-// fp32 master parameters -> packed inference image.
-__global__ void __launch_bounds__(256) k_pack_infer(const float* __restrict__ w, uint16_t* __restrict__ pk) {
+// fp32 master parameters -> packed inference image; base (kNetOut or null)
+// is added to the layer-3 biases (fixed scores under the logits).
+__global__ void __launch_bounds__(256) k_pack_infer(const float* __restrict__ w, const float* __restrict__ base,
+                                                    uint16_t* __restrict__ pk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kNetHid * kNetIn) {  // W1^T[n][k] = W1[k][n]
     const int n = i / kNetIn, k = i % kNetIn;
@@ -66,7 +68,7 @@ __global__ void __launch_bounds__(256) k_pack_infer(const float* __restrict__ w,
     bias[i] = w[kOffB1 + i];
     bias[kNetHid + i] = w[kOffB2 + i];
   }
-  if (i < kL3Rows) bias[2 * kNetHid + i] = i < kNetOut ? w[kOffB3 + i] : 0.0f;
+  if (i < kL3Rows) bias[2 * kNetHid + i] = i < kNetOut ? w[kOffB3 + i] + (base ? base[i] : 0.0f) : 0.0f;
 }
 
 // LDS swizzle of the weight images: chunk c of row r sits at c ^ (r & kSwz).

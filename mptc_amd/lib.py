@@ -6,6 +6,7 @@ is missing -- there is no CPU fallback for any entry point.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import re
 
@@ -32,7 +33,8 @@ class Config(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("wg_size", C.c_int32), ("tile_w", C.c_int32), ("pass_batch", C.c_int32),
-                ("use_graph", C.c_int32), ("win_chunks", C.c_int32), ("prune", C.c_int32)]
+                ("use_graph", C.c_int32), ("win_chunks", C.c_int32), ("prune", C.c_int32),
+                ("ltile_w", C.c_int32)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -119,6 +121,8 @@ def load() -> C.CDLL:
     L.mptc_set_search_mode.argtypes = [_vp, C.c_int32, C.c_int32]
     L.mptc_set_prior.argtypes = [_vp, C.c_int32, _vp]
     L.mptc_get_proposals.argtypes = [_vp, C.c_int32, _vp, C.c_int64]
+    L.mptc_net_set_base.argtypes = [_vp, C.c_int32, _vp]
+    L.mptc_frame_sse.argtypes = [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp]
     L.mptc_encode_frame_noalloc.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                             C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     for name in header_functions():
@@ -245,8 +249,8 @@ class Encoder:
         _check(self.L.mptc_get_tuning(self.h, C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, wg_size=256, tile_w=16, pass_batch=2, use_graph=0, win_chunks=4, prune=1):
-        t = Tuning(wg_size, tile_w, pass_batch, use_graph, win_chunks, prune)
+    def set_tuning(self, wg_size=256, tile_w=16, pass_batch=2, use_graph=0, win_chunks=4, prune=1, ltile_w=8):
+        t = Tuning(wg_size, tile_w, pass_batch, use_graph, win_chunks, prune, ltile_w)
         _check(self.L.mptc_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, frames: list[dict], steps: int = 3):
@@ -317,6 +321,15 @@ class Encoder:
         _check(self.L.mptc_get_proposals(self.h, net, out.ctypes.data, n_blocks))
         return out.reshape(n_blocks, top_k)
 
+    def net_set_base(self, net: int, scores: np.ndarray | None):
+        """Fixed scores added under the network's logits (None removes them)."""
+        if scores is None:
+            _check(self.L.mptc_net_set_base(self.h, net, None))
+        else:
+            a = np.ascontiguousarray(scores, np.float32)
+            assert a.shape == (256,)
+            _check(self.L.mptc_net_set_base(self.h, net, a.ctypes.data))
+
     def set_prior(self, net: int, scores: np.ndarray | None):
         if scores is None:
             _check(self.L.mptc_set_prior(self.h, net, None))
@@ -349,6 +362,19 @@ def encode_frame_noalloc(device: int, d_rgb, w: int, h: int, search_area: int, e
     _check(load().mptc_encode_frame_noalloc(device, _ptr(d_rgb), w, h, search_area, err_threshold,
                                             int(bool(is_intra)), _ptr(d_ref), _ptr(d_phys), _ptr(d_motion),
                                             _ptr(d_unique), _ptr(d_n_unique), _ptr(d_eps), _ptr(d_kind), stream))
+
+
+def frame_sse(d_rgb, w: int, h: int, d_phys, d_out, stream: int | None = None) -> None:
+    """Queues mptc_frame_sse: the frame's squared error against the decoding
+    of its physical blocks, into d_out (device uint64)."""
+    _check(load().mptc_frame_sse(_ptr(d_rgb), w, h, _ptr(d_phys), _ptr(d_out), stream))
+
+
+def psnr_from_sse(sse: int, w: int, h: int) -> float:
+    """PSNR in dB of a frame from its squared error (DXTImage::PSNR,
+    codec/dxt_image.cpp:363-383: mean over pixels, peak 3 * 255^2)."""
+    mse = float(sse) / float(w * h)
+    return 10.0 * math.log10(3.0 * 255.0 * 255.0 / mse) if mse > 0 else float("inf")
 
 
 def next_intra(set_intra: bool, num: int, interval: int):

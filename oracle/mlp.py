@@ -70,8 +70,9 @@ def _r(x, exact):
     return np.asarray(x, np.float64) if exact else bf16(np.asarray(x, np.float32)).astype(np.float64)
 
 
-def forward(params: np.ndarray, px: np.ndarray, exact: bool = False) -> dict:
-    """px: n x 48 uint8 (or, with exact, any real inputs already divided by 255)."""
+def forward(params: np.ndarray, px: np.ndarray, exact: bool = False, base=None) -> dict:
+    """px: n x 48 uint8 (or, with exact, any real inputs already divided by 255).
+    base: optional 256 fixed scores added to the logit columns (not learned)."""
     P = unpack(params)
     if exact:
         X = np.asarray(px, np.float64)
@@ -82,6 +83,8 @@ def forward(params: np.ndarray, px: np.ndarray, exact: bool = False) -> dict:
     H1 = _r(np.maximum(X @ W1 + b1, 0.0), exact)
     H2 = _r(np.maximum(H1 @ W2 + b2, 0.0), exact)
     Y = H2 @ W3 + b3
+    if base is not None:
+        Y[:, :N_LOGITS] += np.asarray(base, np.float32).astype(np.float64)
     return dict(X=X, H1=H1, H2=H2, Y=Y, W1=W1, W2=W2, W3=W3)
 
 
@@ -100,9 +103,9 @@ def target_sets(target, n: int) -> np.ndarray:
 
 
 def loss_grad(params: np.ndarray, px: np.ndarray, target, eps: np.ndarray,
-              seed_w: float, exact: bool = False):
+              seed_w: float, exact: bool = False, base=None):
     """Mean loss over the rows and its gradient (flat, parameter layout)."""
-    f = forward(params, px, exact)
+    f = forward(params, px, exact, base)
     Y = f["Y"]
     n = Y.shape[0]
     logits = Y[:, :N_LOGITS]

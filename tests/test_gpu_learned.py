@@ -423,3 +423,78 @@ def test_learned_1080p_with_network_inference(env, orc):
         assert s["n_blocks"] == 129600
         assert s["inter_candidates"] <= 8 * s["n_blocks"] and s["intra_candidates"] <= 8 * s["n_blocks"]
     enc.close()
+
+
+def test_base_only_network_proposes_like_the_prior(env, orc):
+    """A network whose logit weights and biases are zero, under base scores
+    equal to a prior, proposes exactly what that prior proposes (the start of
+    training on top of a fixed ranking); the frames it encodes are identical."""
+    from oracle import mlp
+    hip, lib = env
+    spec = dict(SEQUENCES["c1_s8_t8"])
+    frames = lib.synth_frames(synth_cfg(spec), 0, 3)
+    S, thr, k = spec["S"], spec["thr"], 8
+    outs = []
+    for use_base in (False, True):
+        seq = Seq(hip, frames)
+        enc = lib.Encoder(seq.w, seq.h, S, thr)
+        for net in (0, 1):
+            if use_base:
+                P = mlp.unpack(mlp.he_params(40 + net))
+                P["W3"][:, :256] = 0.0
+                P["b3"][:256] = 0.0
+                enc.net_set_params(net, mlp.pack(P["W1"], P["b1"], P["W2"], P["b2"], P["W3"], P["b3"]).astype(np.float32))
+                enc.net_set_base(net, nearest_prior(net))
+            else:
+                enc.set_prior(net, nearest_prior(net))
+        enc.set_search_mode(2, k)
+        stats = encode(enc, lib, seq, 3, check=(orc, S, thr) if use_base else None)
+        props = [enc.proposals(net, seq.nb, k) for net in (0, 1)]
+        outs.append(([seq.phys[f].download(np.uint64, seq.nb) for f in range(seq.n)],
+                     [seq.eps[f].download(np.uint8, 6 * seq.nb) for f in range(seq.n)],
+                     [seq.motion[f].download(np.uint8, 2 * seq.nb) for f in range(seq.n)], props,
+                     [(s["n_inter"], s["n_intra"], s["n_unique"]) for s in stats]))
+        enc.close()
+    a, b = outs
+    for f in range(len(frames)):
+        assert np.array_equal(a[0][f], b[0][f]) and np.array_equal(a[1][f], b[1][f]), f"frame {f}"
+        assert np.array_equal(a[2][f], b[2][f]), f"frame {f}"
+    assert np.array_equal(a[3][0], b[3][0]) and np.array_equal(a[3][1], b[3][1])
+    assert a[4] == b[4]
+
+
+@pytest.mark.parametrize("name,thr", [("c1_s8_t8", 8), ("c1_s8_t0", 0)])
+def test_learned_intra_tile_shapes_agree(env, orc, name, thr):
+    """The learned intra fixpoint settles chains inside 8x8, 16x4, 32x2 or
+    64x1 block tiles per pass; the frames do not depend on the tile shape."""
+    hip, lib = env
+    spec = dict(SEQUENCES[name])
+    frames = lib.synth_frames(synth_cfg(spec), 0, 4)
+    S = spec["S"]
+    outs = {}
+    for ltw in (8, 16, 32, 64):
+        log(f"learned intra tile {ltw} x {64 // ltw}, {name} thr={thr}")
+        seq = Seq(hip, frames)
+        enc = lib.Encoder(seq.w, seq.h, S, thr)
+        enc.set_prior(0, nearest_prior(0))
+        enc.set_prior(1, nearest_prior(1))
+        enc.set_search_mode(2, 8)
+        enc.set_tuning(ltile_w=ltw)
+        assert enc.get_tuning()["ltile_w"] == ltw
+        stats = encode(enc, lib, seq, 3, check=(orc, S, thr) if ltw == 32 else None)
+        outs[ltw] = ([seq.phys[f].download(np.uint64, seq.nb) for f in range(seq.n)],
+                     [seq.eps[f].download(np.uint8, 6 * seq.nb) for f in range(seq.n)],
+                     [seq.motion[f].download(np.uint8, 2 * seq.nb) for f in range(seq.n)],
+                     [(s["n_inter"], s["n_intra"], s["n_unique"]) for s in stats])
+        enc.close()
+    for ltw in (16, 32, 64):
+        for f in range(len(frames)):
+            for part in range(3):
+                assert np.array_equal(outs[8][part][f], outs[ltw][part][f]), (ltw, f, part)
+        assert outs[8][3] == outs[ltw][3]
+    with pytest.raises(lib.MptcError):
+        e = lib.Encoder(64, 64, 8, 0)
+        try:
+            e.set_tuning(ltile_w=12)
+        finally:
+            e.close()

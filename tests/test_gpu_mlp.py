@@ -156,3 +156,41 @@ def test_training_fits_a_learnable_rule(env):
     assert np.all(np.isfinite(losses))
     assert losses[:5].mean() > 4.0
     assert losses[-20:].mean() < 1.0
+
+
+def test_base_scores_add_under_the_logits(env):
+    """Fixed base scores: the forward pass and the loss/gradients equal the
+    oracle's with the same base; removing the base restores the plain net."""
+    hip, lib, enc = env
+    n = 700
+    rng = np.random.default_rng(21)
+    p = mlp.he_params(21)
+    base = rng.normal(0, 3, 256).astype(np.float32)
+    enc.net_set_params(0, p)
+    enc.net_set_base(0, base)
+    px = rng.integers(0, 256, (n, 48), dtype=np.uint8)
+    d_px = hip.Buffer.from_array(px)
+    d_y = hip.Buffer(n * 262 * 4)
+    enc.net_forward(0, d_px, n, d_y)
+    y = d_y.download(np.float32, n * 262).reshape(n, 262)
+    ref = mlp.forward(p, px, base=base)["Y"]
+    err = np.abs(y - ref)
+    assert err.max() <= 2e-2 * np.abs(ref).max(), err.max()
+    assert err.mean() <= 1e-3 * np.abs(ref).mean(), err.mean()
+    tgt = set_words(rng, n, 0.05)
+    eps = rng.integers(0, 256, (n, 6), dtype=np.uint8)
+    g, loss = enc.net_gradients(0, d_px, hip.Buffer.from_array(tgt), hip.Buffer.from_array(eps), n, 1.0)
+    rl, rg = mlp.loss_grad(p, px, tgt, eps, 1.0, base=base)
+    assert abs(loss - rl) <= 1e-3 * abs(rl), (loss, rl)
+    # a plain net would give a different loss: the base is really in it
+    assert abs(mlp.loss_grad(p, px, tgt, eps, 1.0)[0] - rl) > 1e-2 * abs(rl)
+    for name, (a, b) in _blocks(lambda a, b: (a, b)).items():
+        d = np.linalg.norm(g[a:b] - rg[a:b])
+        assert d <= 1e-2 * np.linalg.norm(rg[a:b]) + 1e-9, (name, d, np.linalg.norm(rg[a:b]))
+    enc.net_set_base(0, None)
+    enc.net_forward(0, d_px, n, d_y)
+    y0 = d_y.download(np.float32, n * 262).reshape(n, 262)
+    ref0 = mlp.forward(p, px)["Y"]
+    assert np.abs(y0 - ref0).max() <= 2e-2 * np.abs(ref0).max()
+    assert np.allclose(y[:, :256] - y0[:, :256], base[None, :], atol=1e-3 * (1 + np.abs(base).max()))
+    assert np.array_equal(y[:, 256:], y0[:, 256:])  # seed outputs carry no base

@@ -103,9 +103,55 @@ def test_config2_1080p(enc, orc, name):
         want = json.load(fh)[name]
     spec = SEQUENCES[name]
     frames = orc.synth_frames(synth_cfg(spec), len(want))
+    from mptc_amd import hip, lib
+    d_sse = hip.Buffer(8)
     for f, wf in enumerate(want):
         assert orc.psnr_logical(frames[f], res[f]["phys"], res[f]["eps"]) == pytest.approx(wf["psnr_enc"], abs=1e-9)
         assert orc.psnr_physical(frames[f], res[f]["phys"]) == pytest.approx(wf["psnr_dec"], abs=1e-9)
+        # the device-side measure (mptc_frame_sse) gives the reference's decoder PSNR
+        d_rgb, d_phys = hip.Buffer.from_array(frames[f]), hip.Buffer.from_array(res[f]["phys"])
+        lib.frame_sse(d_rgb, spec["width"], spec["height"], d_phys, d_sse)
+        hip.synchronize()
+        sse = int(d_sse.download(np.uint64, 1)[0])
+        assert lib.psnr_from_sse(sse, spec["width"], spec["height"]) == pytest.approx(wf["psnr_dec"], abs=1e-9)
+        d_rgb.free()
+        d_phys.free()
+    d_sse.free()
+
+
+@pytest.mark.parametrize("w,h", [(4, 4), (36, 20), (640, 360)])
+def test_frame_sse_matches_oracle_on_random_words(orc, w, h):
+    """Arbitrary physical words (both colour modes, every index) against the
+    oracle's decoder-side PSNR and a numpy decode of the same words."""
+    from mptc_amd import hip, lib
+    rng = np.random.default_rng(w * h)
+    rgb = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    nb = (w // 4) * (h // 4)
+    phys = rng.integers(0, 2 ** 63, nb, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, nb, dtype=np.uint64)
+    d_rgb, d_phys, d_sse = hip.Buffer.from_array(rgb), hip.Buffer.from_array(phys), hip.Buffer(8)
+    lib.frame_sse(d_rgb, w, h, d_phys, d_sse)
+    hip.synchronize()
+    sse = int(d_sse.download(np.uint64, 1)[0])
+    assert lib.psnr_from_sse(sse, w, h) == pytest.approx(orc.psnr_physical(rgb, phys), abs=1e-9)
+    # numpy decode: 565 endpoints, 4-colour (e1 > e2) or 3-colour + black palettes
+    e1 = (phys & np.uint64(0xFFFF)).astype(np.int64)
+    e2 = ((phys >> np.uint64(16)) & np.uint64(0xFFFF)).astype(np.int64)
+
+    def rgb565(e):
+        r, g, b = e >> 11, (e >> 5) & 63, e & 31
+        return np.stack([(r << 3) | (r >> 2), (g << 2) | (g >> 4), (b << 3) | (b >> 2)], -1)
+
+    c1, c2 = rgb565(e1), rgb565(e2)
+    four = (e1 > e2)[:, None]
+    pal2 = np.where(four, (2 * c1 + c2) // 3, (c1 + c2) // 2)
+    pal3 = np.where(four, (c1 + 2 * c2) // 3, 0)
+    pal = np.stack([c1, c2, pal2, pal3], 1)  # nb x 4 x 3
+    idx = ((phys >> np.uint64(32))[:, None] >> (2 * np.arange(16, dtype=np.uint64))) & np.uint64(3)
+    dec = pal[np.arange(nb)[:, None], idx.astype(np.int64)]  # nb x 16 x 3
+    px = rgb.reshape(h // 4, 4, w // 4, 4, 3).transpose(0, 2, 1, 3, 4).reshape(nb, 16, 3).astype(np.int64)
+    assert sse == int(((px - dec) ** 2).sum())
+    for b in (d_rgb, d_phys, d_sse):
+        b.free()
 
 
 @pytest.mark.parametrize("name", ["c3_t0", "c3_t8", "c3_t32", "c3_t128"])
