@@ -50,6 +50,14 @@ NET_FLOP_L3 = 2 * 256 * 262
 TRAIN_SEEDS = list(range(1000, 1007))  # 7 x 32 frames of 1080p
 
 
+_T0 = time.perf_counter()
+
+
+def note(msg: str):
+    """A progress line on stderr (the JSON result goes to stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:6.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +69,7 @@ def parse():
     ap.add_argument("--config-steps", type=int, default=4, help="timed frames per config 3 / config 4 run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--top-k", type=int, default=8)
+    ap.add_argument("--no-threshold8", action="store_true", help="skip the learned frontier at threshold 8")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--workload", default=WORKLOAD)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -152,7 +161,7 @@ def cpu_baseline(spec, steps):
 
 
 KERNELS = ("fit", "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine", "intra_apply",
-           "intra_reeval", "intra_select", "commit", "unique", "net_input", "net_l12", "net_l3",
+           "intra_reeval", "intra_tile", "commit", "unique", "net_input", "net_l12", "net_l3",
            "propose_inter", "top_intra", "lintra_pass", "pruned_inter", "pruned_intra", "fold_intra")
 
 
@@ -354,10 +363,18 @@ def work_of(stats):
             "max_intra_iterations": max(s["intra_iterations"] for s in stats) if stats else 0}
 
 
-def train_networks(hip, lib, enc, spec, train_steps):
-    """Acceptance sets recorded by the exhaustive search on 1080p training
-    sequences (seeds other than the benchmark's), then GPU training of both
-    networks."""
+def nearest_scores(net: int) -> np.ndarray:
+    """Fixed nearest-first scores over the 16 x 16 offsets of a network's
+    window (inter: dx, dy in -8..7; intra: dx in -8..7, dy in 0..15)."""
+    L = np.arange(256)
+    dx = L % 16 - 8
+    dy = L // 16 - 8 if net == 0 else L // 16
+    return (-(dx * dx + dy * dy) - 1e-3 * L).astype(np.float32)
+
+
+def collect_pairs(hip, lib, enc, spec, seeds, n_frames=32):
+    """Acceptance sets recorded by the exhaustive search (at the context's
+    threshold) on 1080p sequences of the given seeds."""
     from mptc_amd.workloads import synth_cfg
     W, H = spec["width"], spec["height"]
     nb = (W // 4) * (H // 4)
@@ -369,14 +386,13 @@ def train_networks(hip, lib, enc, spec, train_steps):
     t0 = time.perf_counter()
     enc.set_search_mode(lib.SEARCH_EXHAUSTIVE, 8)
     enc.record_pairs(True)
-    n_frames = 0
-    for seed in TRAIN_SEEDS:
+    for seed in seeds:
         cfg = dict(spec)
         cfg["seed"] = seed
-        frames = lib.synth_frames(synth_cfg(cfg), 0, 32)
+        frames = lib.synth_frames(synth_cfg(cfg), 0, n_frames)
         d_fr = hip.Buffer.from_array(frames)
         set_intra, num = True, 1
-        for f in range(32):
+        for f in range(n_frames):
             rgb = d_fr.offset(f * W * H * 3)
             enc.encode_frame_device(rgb, W, H, set_intra, None if set_intra else d_phys[(f + 1) % 2], d_phys[f % 2],
                                     d_motion, d_unique, d_nu, d_eps, None)
@@ -384,26 +400,46 @@ def train_networks(hip, lib, enc, spec, train_steps):
                 b = bufs[net]
                 enc.collect_pairs(net, b["px"], b["tgt"], b["eps"], b["count"], cap)
             set_intra, num = lib.next_intra(set_intra, num, spec["intra_interval"])
-        n_frames += 32
         d_fr.free()
     enc.record_pairs(False)
-    t_collect = time.perf_counter() - t0
-    n_pairs = {net: min(cap, int(bufs[net]["count"].download(np.uint32, 1)[0])) for net in (0, 1)}
-    info = {"training_frames": n_frames, "training_seeds": TRAIN_SEEDS, "pairs_inter": n_pairs[0],
-            "pairs_intra": n_pairs[1], "collect_s": round(t_collect, 2)}
+    hip.synchronize()
+    for buf in d_phys + [d_motion, d_unique, d_nu, d_eps]:
+        buf.free()
     for net in (0, 1):
-        enc.net_init(net, 11 + net)
-        b = bufs[net]
-        t1 = time.perf_counter()
-        losses = enc.net_train(net, b["px"], b["tgt"], b["eps"], n_pairs[net], batch=4096, steps=train_steps,
-                               lr=2e-3, seed_weight=1.0, seed=7, lr_final=1e-4)
-        key = "inter" if net == 0 else "intra"
-        info[f"train_s_{key}"] = round(time.perf_counter() - t1, 2)
-        info[f"loss_{key}"] = [round(float(losses[:20].mean()), 4), round(float(losses[-50:].mean()), 4)]
+        bufs[net]["n"] = min(cap, int(bufs[net]["count"].download(np.uint32, 1)[0]))
+    info = {"training_frames": n_frames * len(seeds), "training_seeds": list(seeds),
+            "pairs_inter": bufs[0]["n"], "pairs_intra": bufs[1]["n"],
+            "collect_s": round(time.perf_counter() - t0, 2)}
+    return bufs, info
+
+
+def free_pairs(bufs):
     for b in bufs.values():
         for v in b.values():
             if hasattr(v, "free"):
                 v.free()
+
+
+def train_networks(lib, enc, bufs, train_steps, prior_alpha):
+    """GPU training of both networks on recorded pairs.  With prior_alpha > 0
+    the logits are learned as corrections on top of fixed base scores
+    prior_alpha x the nearest-first ranking: the logit columns of W3 and b3
+    start at zero, so training starts from that ranking."""
+    info = {"prior_alpha": prior_alpha}
+    for net in (0, 1):
+        enc.net_init(net, 11 + net)
+        if prior_alpha > 0:
+            enc.net_set_params(net, lib.zero_logit_layer(enc.net_params(net)))
+            enc.net_set_base(net, prior_alpha * nearest_scores(net))
+        else:
+            enc.net_set_base(net, None)
+        b = bufs[net]
+        t1 = time.perf_counter()
+        losses = enc.net_train(net, b["px"], b["tgt"], b["eps"], b["n"], batch=4096, steps=train_steps,
+                               lr=2e-3, seed_weight=1.0, seed=7, lr_final=1e-4)
+        key = "inter" if net == 0 else "intra"
+        info[f"train_s_{key}"] = round(time.perf_counter() - t1, 2)
+        info[f"loss_{key}"] = [round(float(losses[:20].mean()), 4), round(float(losses[-50:].mean()), 4)]
     return info
 
 
@@ -441,9 +477,14 @@ def measure_config3(hip, lib, steps, warmup):
     enc = lib.Encoder(W, H, spec0["S"], spec0["thr"])
     R = Runner(hip, lib, enc, spec0, frames, max(steps, warmup, 2))
     out = {}
+    # intra fixpoint passes per threshold, the faster of itile_w 0/8/16/32/64
+    # measured on MI355X at this size: tile-resolved 64 x 1 passes at
+    # threshold 8, plain re-fold passes elsewhere
+    itile = {8: 64}
     for name in names:
         spec = SEQUENCES[name]
         enc.set_params(spec["S"], spec["thr"])
+        enc.set_tuning(**dict(enc.get_tuning(), itile_w=itile.get(spec["thr"], 0)))
         R.spec = spec
         el = R.timed(steps, warmup)
         ok, n = R.verify(steps, load_fixture(name))
@@ -456,7 +497,7 @@ def measure_config3(hip, lib, steps, warmup):
                      "threshold": spec["thr"], "bit_exact": ok, "bit_exact_frames_checked": n, "roofline": roof,
                      "kernels": {x: k[x] for x in k if k[x]["total_ms"] > 0.01 * steps},
                      "candidates_per_frame": round((w["inter_candidates"] + w["intra_candidates"]) / steps),
-                     "max_intra_iterations": w["max_intra_iterations"]}
+                     "max_intra_iterations": w["max_intra_iterations"], "itile_w": enc.get_tuning()["itile_w"]}
     enc.close()
     return {"workload": f"config 3: {W}x{H} RGB8, S={spec0['S']} (64x64-block window), seeded generator, "
                         f"{steps} timed frames from frame 0 per threshold", "blocks_per_frame": nb, "results": out}
@@ -535,6 +576,111 @@ def measure_config4(hip, lib, steps, warmup):
             "bit_exact_frames_checked": checked, "kernels": k}
 
 
+def learned_point(lib, enc, R, a, k, plain, with_quality=True):
+    """Timed learned-mode encode of the workload at top-k = k: throughput and
+    what it keeps of the plain search's reuse (plain: that search's value and
+    work at the same threshold)."""
+    nb = R.nb
+    enc.set_search_mode(lib.SEARCH_LEARNED, k)
+    t = R.timed(a.steps, a.warmup)
+    w = work_of(R.run(a.steps, collect=True))
+    v = a.steps * nb / t / 1e6
+    reused_plain = plain["work"]["n_inter"] + plain["work"]["n_intra"]
+    reused = w["n_inter"] + w["n_intra"]
+    pt = {"top_k": k, "value": round(v, 4), "ms_per_step": round(t / a.steps * 1e3, 4),
+          "speedup_vs_plain": round(v / plain["value"], 3), "reused_blocks": reused,
+          "n_inter": w["n_inter"], "n_intra": w["n_intra"],
+          "reuse_recall": round(reused / max(1, reused_plain), 4),
+          "unique_blocks_delta_pct": round(100.0 * (w["n_unique"] - plain["work"]["n_unique"])
+                                           / max(1, plain["work"]["n_unique"]), 3)}
+    if with_quality:
+        q = R.quality(a.steps)
+        pt["psnr_db"] = round(q, 4)
+        pt["psnr_delta_db"] = round(q - plain["psnr_db"], 4)
+    return pt, w
+
+
+def learned_frontier(hip, lib, enc, R, a, plain, prior_alpha):
+    """k in {4, 8, 16} for the trained networks (already set), the fixed
+    nearest-first ranking and untrained networks, in that order."""
+    fr = {}
+    for method in ("trained", "nearest_prior", "untrained"):
+        if method == "nearest_prior":
+            for net in (0, 1):
+                enc.set_prior(net, nearest_scores(net))
+        elif method == "untrained":
+            for net in (0, 1):
+                enc.set_prior(net, None)
+                enc.net_init(net, 99 + net)
+                if prior_alpha > 0:  # the same start as training: the base ranking, untrained corrections
+                    enc.net_set_params(net, lib.zero_logit_layer(enc.net_params(net)))
+                    enc.net_set_base(net, prior_alpha * nearest_scores(net))
+        fr[method] = {str(k): learned_point(lib, enc, R, a, k, plain)[0] for k in (4, 8, 16)}
+    return fr
+
+
+PRIOR_ALPHA = 0.25  # base scores = 0.25 x nearest-first ranking (corrections learned on top)
+
+
+def measure_learned(hip, lib, enc, R, spec, a, value, work, psnr_plain, tuning):
+    """The learned-proposal path at the workload's threshold (trained on pairs
+    of other seeds, launch shape autotuned in learned mode), its kernels and
+    MFMA roofline, and the k frontier against the nearest-first ranking and
+    untrained networks; then the same at threshold 8 (the c2_t8 sequence:
+    the same frames), where reuse costs PSNR."""
+    nb = R.nb
+    plain = {"value": value, "work": work, "psnr_db": psnr_plain}
+    note("learned: collecting pairs and training")
+    bufs, info = collect_pairs(hip, lib, enc, spec, TRAIN_SEEDS)
+    info.update(train_networks(lib, enc, bufs, a.train_steps, PRIOR_ALPHA))
+    free_pairs(bufs)
+    learned = {"top_k": a.top_k, "training": info}
+    enc.set_search_mode(lib.SEARCH_LEARNED, a.top_k)
+    if not a.no_autotune:
+        ltune, _ = R.autotune()
+        learned["tuning"] = ltune
+    pt, lw = learned_point(lib, enc, R, a, a.top_k, plain)
+    learned.update(pt)
+    lk, lms_ev = R.kernels(a.steps)
+    learned.update({"kernels": lk, "ms_per_step_with_kernel_events": round(lms_ev, 4), "work": lw})
+    # MFMA roofline of the network's layer-1+2 kernel (bf16 dense)
+    if "net_l12" in lk:
+        ach = nb * NET_FLOP_L12 / (lk["net_l12"]["avg_ms"] * 1e-3) / 1e12
+        learned["roofline_net_l12"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": MFMA_BF16_PEAK,
+                                       "unit": "TFLOP/s", "frac": round(ach / MFMA_BF16_PEAK, 4),
+                                       "work_per_launch": f"{nb} blocks x {NET_FLOP_L12} flop"}
+    if "net_l3" in lk:
+        ach = nb * NET_FLOP_L3 / (lk["net_l3"]["avg_ms"] * 1e-3) / 1e12
+        learned["roofline_net_l3"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": MFMA_BF16_PEAK,
+                                      "unit": "TFLOP/s", "frac": round(ach / MFMA_BF16_PEAK, 4),
+                                      "work_per_launch": f"{nb} blocks x {NET_FLOP_L3} flop (logits) + top-k"}
+    note("learned: k frontier")
+    learned["frontier"] = learned_frontier(hip, lib, enc, R, a, plain, PRIOR_ALPHA)
+    if a.no_threshold8:
+        return learned
+    note("learned: threshold 8")
+    spec8 = dict(spec)
+    spec8["thr"] = 8
+    enc.set_search_mode(lib.SEARCH_EXHAUSTIVE, 8)
+    enc.set_params(spec["S"], 8)
+    enc.set_tuning(**tuning)
+    t8 = R.timed(a.steps, a.warmup)
+    w8 = work_of(R.run(a.steps, collect=True))
+    plain8 = {"value": a.steps * nb / t8 / 1e6, "work": w8, "psnr_db": R.quality(a.steps)}
+    bufs, info8 = collect_pairs(hip, lib, enc, spec8, TRAIN_SEEDS[:4])
+    info8.update(train_networks(lib, enc, bufs, a.train_steps, PRIOR_ALPHA))
+    free_pairs(bufs)
+    if "tuning" in learned:
+        enc.set_tuning(**learned["tuning"])
+    learned["threshold8"] = {
+        "plain": {"value": round(plain8["value"], 4), "reused_blocks": w8["n_inter"] + w8["n_intra"],
+                  "n_unique": w8["n_unique"], "psnr_db": round(plain8["psnr_db"], 4)},
+        "training": info8,
+        "frontier": learned_frontier(hip, lib, enc, R, a, plain8, PRIOR_ALPHA)}
+    enc.set_params(spec["S"], spec["thr"])
+    return learned
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -546,6 +692,7 @@ def main():
     # the CPU baseline first, in worker processes forked before any GPU work
     cpu = None
     if not a.no_cpu_baseline and not a.copy_calibration:
+        note("CPU baseline")
         cpu = cpu_baseline(SEQUENCES[a.workload], a.steps)
 
     from mptc_amd import hip, lib
@@ -568,6 +715,7 @@ def main():
     R = Runner(hip, lib, enc, spec, frames, max(a.steps, a.warmup, 2))
 
     # plain (exhaustive, bit-exact) path, in the launch shape the autotuner picks
+    note("plain path: autotune, timed steps, checks")
     tuning, tune_ms = R.autotune() if not a.no_autotune else (enc.get_tuning(), None)
     elapsed = R.timed(a.steps, a.warmup)
     value = a.steps * nb / elapsed / 1e6
@@ -636,57 +784,15 @@ def main():
     }
 
     if not a.no_learned:
-        info = train_networks(hip, lib, enc, spec, a.train_steps)
-        learned = {"top_k": a.top_k, "training": info}
-        enc.set_search_mode(lib.SEARCH_LEARNED, a.top_k)
-        el = R.timed(a.steps, a.warmup)
-        lk, lms_ev = R.kernels(a.steps)
-        lw = work_of(R.run(a.steps, collect=True))
-        lpsnr = R.quality(a.steps)
-        lval = a.steps * nb / el / 1e6
-        reused_plain = work["n_inter"] + work["n_intra"]
-        reused = lw["n_inter"] + lw["n_intra"]
-        learned.update({
-            "value": round(lval, 4), "ms_per_step": round(el / a.steps * 1e3, 4),
-            "speedup_vs_plain": round(lval / value, 3),
-            "psnr_db": round(lpsnr, 4), "psnr_delta_db": round(lpsnr - psnr_plain, 4),
-            "reused_blocks": reused, "reused_blocks_plain": reused_plain,
-            "reuse_recall": round(reused / max(1, reused_plain), 4),
-            "unique_blocks_delta_pct": round(100.0 * (lw["n_unique"] - work["n_unique"]) / max(1, work["n_unique"]), 3),
-            "kernels": lk, "ms_per_step_with_kernel_events": round(lms_ev, 4), "work": lw,
-        })
-        # MFMA roofline of the network's layer-1+2 kernel (bf16 dense)
-        if "net_l12" in lk:
-            rows = nb  # each launch covers every block of the frame
-            ach = rows * NET_FLOP_L12 / (lk["net_l12"]["avg_ms"] * 1e-3) / 1e12
-            learned["roofline_net_l12"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": MFMA_BF16_PEAK,
-                                           "unit": "TFLOP/s", "frac": round(ach / MFMA_BF16_PEAK, 4),
-                                           "work_per_launch": f"{rows} blocks x {NET_FLOP_L12} flop"}
-        # ablation: the same search with fixed nearest-first scores, and with untrained networks
-        Ls = np.arange(256)
-        abl = {}
-        for name in ("nearest_prior", "untrained"):
-            if name == "nearest_prior":
-                for net in (0, 1):
-                    dx = Ls % 16 - 8
-                    dy = Ls // 16 - 8 if net == 0 else Ls // 16
-                    enc.set_prior(net, (-(dx * dx + dy * dy) - 1e-3 * Ls).astype(np.float32))
-            else:
-                for net in (0, 1):
-                    enc.set_prior(net, None)
-                    enc.net_init(net, 99 + net)
-            e2 = R.timed(a.steps, a.warmup)
-            w2 = work_of(R.run(a.steps, collect=True))
-            abl[name] = {"value": round(a.steps * nb / e2 / 1e6, 2),
-                         "reused_blocks": w2["n_inter"] + w2["n_intra"],
-                         "n_inter": w2["n_inter"], "n_intra": w2["n_intra"],
-                         "psnr_db": round(R.quality(a.steps), 4)}
-        learned["ablation"] = abl
-        out["learned"] = learned
+        note("learned path")
+        out["learned"] = measure_learned(hip, lib, enc, R, spec, a, value, work, psnr_plain, tuning)
         enc.set_search_mode(lib.SEARCH_EXHAUSTIVE, 8)
+        enc.set_tuning(**tuning)
 
     if not a.no_configs:
+        note("config 3 (4K, S=32, threshold sweep)")
         out["config3"] = measure_config3(hip, lib, steps=a.config_steps, warmup=1)
+        note("config 4 (12 uneven streams)")
         out["config4"] = measure_config4(hip, lib, steps=a.config_steps, warmup=1)
 
     if cpu is not None:
@@ -696,6 +802,7 @@ def main():
         out["cpu_host"] = host
         out["gpu_over_cpu"] = round(value / allc["value"], 1)
         out["gpu_over_cpu_1core"] = round(value / one["value"], 1)
+    note("done")
     print(json.dumps(out), flush=True)
     enc.close()
 

@@ -61,8 +61,10 @@ typedef struct {
   int32_t prune;      /* 1: for S <= 8 and thresholds 0..4, evaluate exactly only the
                          candidates an error lower bound cannot rule out (same output);
                          0: evaluate every candidate */
-  int32_t ltile_w;    /* learned intra search: a wave settles reuse chains inside a tile of
-                         ltile_w x 64/ltile_w blocks per pass; 8, 16, 32 or 64 */
+  int32_t itile_w;    /* intra fixpoint passes: 8, 16, 32 or 64: a wave settles the reuse
+                         chains inside a tile of itile_w x 64/itile_w blocks per pass;
+                         0: exhaustive search re-folds flagged blocks once per pass, learned
+                         search uses 8 x 8 tiles (same output for every value) */
 } mptc_tuning;
 
 /* One frame of a batched step: device pointers, layouts as mptc_reencode. */
@@ -147,8 +149,8 @@ int32_t mptc_encode_batch_device(mptc_ctx* ctx, int32_t n_frames, const mptc_fra
                                  void* hip_stream);
 
 /* Launch-shape tuning.  mptc_autotune times every candidate shape (wg_size x
- * tile_w x win_chunks x use_graph, prune 0/1 where the pruned search applies,
- * ltile_w in the learned intra search) on
+ * tile_w x win_chunks x use_graph, prune 0/1 where the pruned search applies;
+ * then itile_w 0/8/16/32/64 on the winner) on
  * `steps` encodes of the given batch (outputs are overwritten), keeps the
  * fastest and reports it. */
 int32_t mptc_get_tuning(mptc_ctx* ctx, mptc_tuning* out);
@@ -170,7 +172,7 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
  * accumulated launch count and milliseconds of kernel `name` ("fit": the step's
  * first kernel: counters cleared, fit, reference tables, network input),
  * "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine",
- * "intra_apply", "intra_reeval", "intra_select", "commit", "unique";
+ * "intra_apply", "intra_reeval", "intra_tile", "commit", "unique";
  * learned search: "net_input", "net_l12", "net_l3" (inference), "propose_inter",
  * "top_intra", "lintra_pass"; "net_forward": training products; pruned search:
  * "pruned_inter", "pruned_intra", "fold_intra") since it was enabled

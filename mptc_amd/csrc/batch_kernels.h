@@ -13,8 +13,10 @@
 //                  deltas E, folds the selection, lists the words that change
 //   k_intra_apply  the listed words become current (first pass only)
 //   fixpoint passes, each proportional to the work left:
-//     k_intra_reeval (block, position) pairs that see a changed word
-//     k_intra_select re-folds those blocks; changed words become current
+//     k_intra_reeval    (block, position) pairs that see a changed word
+//     k_intra_select    re-folds those blocks; changed words become current
+//     or k_intra_tile_pass, which also settles the reuse chains inside each
+//                       64-block tile (tuning itile_w)
 //   The intra words are the unique fixpoint of a raster-triangular system
 //   (each block's window holds only earlier blocks); that fixpoint is the
 //   reference's sequential raster-order result.  Passes stop when no word
@@ -50,8 +52,8 @@ struct FrameDev {
   int32_t n_tiles;
   int32_t wtile_base;      // first wave tile of the search kernels
   int32_t tiles_x;         // wave tiles per row of wave tiles
-  int32_t ltile_base;      // first 8x8 tile of the learned intra passes
-  int32_t ltiles_x;        // 8x8 tiles per row
+  int32_t itile_base;      // first tile of the intra fixpoint passes (64 blocks each)
+  int32_t itiles_x;        // those tiles per row
 };
 
 // Statistics counters are spread over kStatSlots copies (slot = workgroup
@@ -66,6 +68,7 @@ struct DevStats {
   unsigned int n_unique, active_passes;  // passes that changed a word
   unsigned int n_changed[2];             // length of the changed-word list, by pass parity
   unsigned int pass_done;                // workgroups of the running fixpoint pass that finished
+  unsigned int skip_tile;                // the last pass was tile-resolved (k_intra_tile_pass)
 };
 
 // The pruning bound's table of an index word (prune_kernels.h).
@@ -619,7 +622,9 @@ __global__ void __launch_bounds__(256) k_intra_apply(const uint32_t* __restrict_
 
 // Fixpoint pass, part 1, over work items w = w0, w0 + stride, ... < n_chg *
 // 4S^2: every (block, position) pair whose intra window holds a word changed
-// in the previous step is re-evaluated with the new word.  Work item =
+// in the previous step is re-evaluated with the new word, except (skip_tw_log
+// >= 0) pairs inside one tile of 2^skip_tw_log x 64 / 2^skip_tw_log blocks,
+// which the previous pass, tile-resolved, evaluated with the final word.  Work item =
 // (changed position, slot); when 64 divides the 4S^2 slots every wave serves
 // one changed position.  Blocks whose fold may change are flagged for part 2
 // (plain stores: racing writers all store 1), and the passing sets pm (when
@@ -629,7 +634,8 @@ __device__ __forceinline__ unsigned long long reeval_items(
     const uint64_t* __restrict__ own, const uint8_t* __restrict__ need, const uint32_t* __restrict__ cw,
     const float4* __restrict__ ctab, const int* __restrict__ chg_list, unsigned n_chg, uint2* __restrict__ E4,
     size_t nbs, uint32_t* __restrict__ dirty, const int* __restrict__ orig, const uint64_t* __restrict__ dec,
-    uint32_t* __restrict__ pm, const uint8_t* s_snap, const float2* s_ab, size_t w0, size_t stride) {
+    uint32_t* __restrict__ pm, const uint8_t* s_snap, const float2* s_ab, size_t w0, size_t stride,
+    int skip_tw_log) {
   const int W2 = 2 * S, slots = W2 * W2;
   const size_t total = (size_t)n_chg * slots;
   unsigned long long n_eval = 0;
@@ -647,6 +653,10 @@ __device__ __forceinline__ unsigned long long reeval_items(
     const int qx = ql % d.bw, qy = ql / d.bw;
     const int by = qy + r, bx = qx - S + 1 + t;
     if ((r == 0 && t < S) || by >= d.bh || bx < 0 || bx >= d.bw) continue;
+    // pairs inside one tile of the last (tile-resolved) pass are current
+    if (skip_tw_log >= 0 && (qx >> skip_tw_log) == (bx >> skip_tw_log) &&
+        (qy >> (6 - skip_tw_log)) == (by >> (6 - skip_tw_log)))
+      continue;
     const int k = d.base + by * d.bw + bx;
     if (!need[k]) continue;
     BlockPx px;
@@ -727,9 +737,11 @@ __global__ void __launch_bounds__(256) k_intra_reeval(const FrameDev* __restrict
                                                       const float4* __restrict__ ctab, const int* __restrict__ chg_list,
                                                       uint2* __restrict__ E4, size_t nbs, uint32_t* __restrict__ dirty,
                                                       const int* __restrict__ orig, const uint64_t* __restrict__ dec,
-                                                      DevStats* __restrict__ st, int par, uint32_t* __restrict__ pm) {
+                                                      DevStats* __restrict__ st, int par, uint32_t* __restrict__ pm,
+                                                      int tw_log) {
   const int slots = 4 * S * S;
   const unsigned n_chg = st->n_changed[par];
+  const int skip = st->skip_tile ? tw_log : -1;
   if (blockIdx.x == 0 && threadIdx.x == 0) st->n_changed[par ^ 1] = 0;
   if ((size_t)blockIdx.x * blockDim.x >= (size_t)n_chg * slots) return;  // workgroup-uniform
   __shared__ uint8_t s_snap[512];
@@ -739,7 +751,7 @@ __global__ void __launch_bounds__(256) k_intra_reeval(const FrameDev* __restrict
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&st->active_passes, 1u);
   const unsigned long long n_eval =
       reeval_items(fr, sid, S, thr, own, need, cw, ctab, chg_list, n_chg, E4, nbs, dirty, orig, dec, pm, s_snap, s_ab,
-                   (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
+                   (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x, skip);
   wg_count(&s_acc, n_eval, st->reeval_cands);
 }
 
@@ -764,6 +776,152 @@ __global__ void __launch_bounds__(256) k_intra_select(const FrameDev* __restrict
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool changed = k < total && select_block(fr, sid, k, S, thr, own, cw, ctab, E4, nbs, dirty, dec, pm);
   wg_append(changed, k, chg_list, &st->n_changed[par], &s_base);
+  pass_end(st, par, nullptr, conv);
+}
+
+// Fixpoint pass, part 2, tile-resolved: a wave
+// owns a tile of 2^tw_log x 64 / 2^tw_log blocks of one frame (lane = row *
+// width + column).  Round 0 re-folds the blocks k_intra_reeval flagged.
+// Then, while a word of the tile changed in the last round, every block whose
+// window holds such a word re-evaluates that position with it (taken from the
+// owning lane), updates E and its passing set, and re-folds when a passing
+// delta appeared, vanished or changed or the position is the one it reuses.
+// A window holds only earlier blocks in raster order, which inside a tile are
+// lower lanes: lane l is final after round l, so the rounds end by round 64.
+// The tile's words are written once, after the rounds (other tiles read the
+// word a block started the pass with or the one it ends with), and those that
+// changed are listed for the next pass, whose k_intra_reeval skips the pairs
+// inside one tile (settled here).  Writes the changed list of parity `par`;
+// its length goes to *conv when the pass ends.
+constexpr int kTileRounds = 66;
+
+__global__ void __launch_bounds__(256) k_intra_tile_pass(
+    const FrameDev* __restrict__ fr, int n_frames, int n_itiles, int tw_log, int S, int thr,
+    const uint64_t* __restrict__ own, const uint8_t* __restrict__ need, uint32_t* __restrict__ cw,
+    float4* __restrict__ ctab, uint2* __restrict__ E4, size_t nbs, uint32_t* __restrict__ dirty,
+    const int* __restrict__ orig, uint64_t* __restrict__ dec, int* __restrict__ chg_list, DevStats* __restrict__ st,
+    int par, unsigned int* __restrict__ conv, uint32_t* __restrict__ pm) {
+  // The preceding k_intra_reeval flags blocks only from the changed list of
+  // parity par ^ 1; when that list was empty there is nothing to re-fold.
+  // No workgroup of this pass writes that counter, so either every workgroup
+  // returns here or none does; an empty pass reports nothing (pass_end): the
+  // pass that produced the empty list already wrote 0 to *conv.
+  if (st->n_changed[par ^ 1] == 0) return;
+  __shared__ uint8_t s_snap[512];
+  __shared__ float2 s_ab[4];
+  __shared__ unsigned long long s_acc;
+  __shared__ unsigned int s_base;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->skip_tile = 1u;  // read by the next pass's k_intra_reeval
+  const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const int lane = (int)(threadIdx.x & 63);
+  const int W2 = 2 * S;
+  int16_t* E = reinterpret_cast<int16_t*>(E4);
+  unsigned long long n_eval = 0;
+  bool changed = false;
+  int g = 0;
+  // the tile's geometry and its flagged blocks (k_intra_reeval flags only
+  // blocks that search the intra window)
+  const FrameDev* dp = nullptr;
+  int x0 = 0, y0 = 0, bx = 0, by = 0;
+  bool inside = false, redo = false;
+  if (wv < n_itiles) {
+    int f = 0;
+    while (f + 1 < n_frames && fr[f + 1].itile_base <= wv) ++f;
+    dp = &fr[f];
+    const int lt = wv - dp->itile_base, tx = lt % dp->itiles_x, ty = lt / dp->itiles_x;
+    x0 = tx << tw_log;
+    y0 = ty << (6 - tw_log);
+    bx = x0 + (lane & ((1 << tw_log) - 1));
+    by = y0 + (lane >> tw_log);
+    inside = bx < dp->bw && by < dp->bh;
+    g = inside ? dp->base + by * dp->bw + bx : 0;
+    redo = inside && dirty[g] != 0u;
+  }
+  // the evaluation tables, only where some block of the workgroup re-folds
+  if (__syncthreads_or(redo)) load_lds_tables(s_snap, s_ab);
+  if (__ballot(redo) != 0ull) {
+    const FrameDev& d = *dp;
+    const int tw = 1 << tw_log;
+    const bool act = inside && need[g];
+    const uint32_t start = inside ? cw[g] : 0u;
+    uint32_t cur = start;
+    uint64_t dc = act ? dec[g] : 0;
+    bool have_be = false;
+    // the lanes of the tile whose blocks lie in this block's window
+    unsigned long long wmask = 0ull;
+    if (act)
+      for (int L = 0; L < 64; ++L) {
+        const int r = by - (y0 + (L >> tw_log)), t = bx + S - 1 - (x0 + (L & (tw - 1)));
+        if (r >= 0 && r < W2 && t >= 0 && t < W2 && !(r == 0 && t < S)) wmask |= 1ull << L;
+      }
+    BlockEval be;
+    for (int round = 0; round < kTileRounds; ++round) {
+      // fold the flagged blocks; a selected position inside the tile takes
+      // its word from the owning lane
+      int src = -1, sel_i = 0, sel_j = 0;
+      bool acc = false;
+      if (redo) {
+        dirty[g] = 0u;
+        __asm__ volatile("" ::: "memory");  // E and pm: this lane's stores before the fold reads them
+        const Fold fd = pm ? fold_passing_set(pm, E4, nbs, g, S) : fold_passing(E4, nbs, g, S, thr);
+        if (fd.sel >= 0 && fd.min_err <= thr) {
+          acc = true;
+          sel_j = by - fd.sel / W2;
+          sel_i = bx + S - 1 - fd.sel % W2;
+          if (sel_j >= y0 && sel_i >= x0 && sel_i < x0 + tw) src = ((sel_j - y0) << tw_log) | (sel_i - x0);
+        }
+      }
+      const uint32_t v = __shfl(cur, src >= 0 ? src : lane);
+      bool ch = false;
+      if (redo) {
+        const uint32_t w = !acc ? (uint32_t)(own[g] >> 32) : src >= 0 ? v : cw[d.base + sel_j * d.bw + sel_i];
+        dc = acc ? pack_dec(w, (uint32_t)(sel_i - bx + S), (uint32_t)(sel_j - by + W2 - 1), true) : 0;
+        dec[g] = dc;
+        ch = w != cur;
+        cur = w;
+      }
+      const unsigned long long mask = __ballot(ch);
+      if (mask == 0ull) break;
+      // re-evaluate the window positions that hold a word changed this round;
+      // each lane walks its own such positions (a lane reads the word from
+      // the owning lane), so lanes evaluate side by side
+      redo = false;
+      unsigned long long pend = mask & wmask;
+      while (__ballot(pend != 0ull) != 0ull) {
+        const int L = pend != 0ull ? __builtin_ctzll(pend) : lane;
+        const uint32_t w = __shfl(cur, L);
+        if (pend == 0ull) continue;
+        pend &= pend - 1ull;
+        const int r = by - (y0 + (L >> tw_log)), t = bx + S - 1 - (x0 + (L & (tw - 1)));
+        if (!have_be) {
+          BlockPx px;
+          load_block(d.rgb, d.width, bx, by, px);
+          make_block_eval(px, own[g], orig[g], be);
+          have_be = true;
+        }
+        const CandResult res = eval_candidate<false>(be, w, cand_table(w), s_snap, s_ab);
+        const int16_t ev = res.valid ? sat16(res.err) : kNoCand;
+        const int s = r * W2 + t;
+        int16_t* slot = &E[((size_t)(s >> 2) * nbs + g) * 4 + (s & 3)];
+        ++n_eval;
+        const int16_t old = *slot;
+        if (old != ev) *slot = ev;
+        const bool is_sel = dec_acc(dc) && (W2 - 1 - (int)((dc >> 40) & 0xFF)) * W2 +
+                                                   (W2 - 1 - (int)((dc >> 32) & 0xFF)) == s;
+        const bool passes_old = old != kNoCand && old <= thr, passes_new = ev != kNoCand && ev <= thr;
+        if ((old != ev && (passes_old || passes_new)) || is_sel) redo = true;
+        if (pm && passes_old != passes_new) pm[(size_t)g * kPassWords + (s >> 5)] ^= 1u << (s & 31);
+      }
+    }
+    changed = inside && cur != start;
+    if (changed) {
+      cw[g] = cur;
+      const CandTab t = cand_table(cur);
+      ctab[g] = make_float4(t.asq, t.bsq, t.ab, t.f);
+    }
+  }
+  wg_count(&s_acc, n_eval, st->reeval_cands);
+  wg_append(changed, g, chg_list, &st->n_changed[par], &s_base);
   pass_end(st, par, nullptr, conv);
 }
 

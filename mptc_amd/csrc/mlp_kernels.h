@@ -454,7 +454,7 @@ constexpr int kMaxRounds = 512;
 constexpr int kNotEvaluated = (int)0x80000000;
 
 template <int K>
-__global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict__ fr, int n_frames, int n_ltiles,
+__global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict__ fr, int n_frames, int n_itiles,
                                                      int ltw_log, int S, int thr, int kk, const int16_t* __restrict__ prop,
                                                      const uint64_t* __restrict__ own, const uint8_t* __restrict__ need,
                                                      uint32_t* __restrict__ cw, uint32_t* __restrict__ seen,
@@ -480,13 +480,13 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
   const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int lane = (int)(threadIdx.x & 63);
   unsigned long long n_cand = 0, n_re = 0;  // first evaluations, re-evaluations
-  if (wv < n_ltiles) {
+  if (wv < n_itiles) {
     int f = 0;
-    while (f + 1 < n_frames && fr[f + 1].ltile_base <= wv) ++f;
+    while (f + 1 < n_frames && fr[f + 1].itile_base <= wv) ++f;
     const FrameDev& d = fr[f];
     // the wave's tile: ltw x lth blocks, lane = row * ltw + column
     const int ltw = 1 << ltw_log, lth_log = 6 - ltw_log;
-    const int lt = wv - d.ltile_base, tx = lt % d.ltiles_x, ty = lt / d.ltiles_x;
+    const int lt = wv - d.itile_base, tx = lt % d.itiles_x, ty = lt / d.itiles_x;
     const int bx = (tx << ltw_log) + (lane & (ltw - 1)), by = (ty << lth_log) + (lane >> ltw_log);
     const bool inside = bx < d.bw && by < d.bh;
     const int g = inside ? d.base + by * d.bw + bx : 0;

@@ -105,12 +105,12 @@ int init_tables(int device) {
 
 enum KernelId {
   KF_FIT, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
-  KF_SELECT, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
+  KF_TILE, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
   KF_NET_L3, KF_PRUNED_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
 };
 const char* kKernelNames[KF_N] = {"fit",          "inter",
                                   "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
-                                  "intra_apply",  "intra_reeval", "intra_select", "commit",
+                                  "intra_apply",  "intra_reeval", "intra_tile", "commit",
                                   "unique",       "net_input",   "net_forward", "propose_inter",
                                   "top_intra",    "lintra_pass",  "net_l12",     "net_l3",
                                   "pruned_inter", "pruned_intra", "fold_intra"};
@@ -182,7 +182,7 @@ struct mptc_ctx {
   // work on the context's buffers is ordered whatever streams callers use.
   hipEvent_t done_ev = nullptr;
   hipStream_t done_stream = nullptr;  // the stream done_ev was last recorded on (null: never)
-  mptc_tuning tune{256, 16, 2, 0, 4, 1, 8};  // the fastest 1080p shape measured on MI355X
+  mptc_tuning tune{256, 16, 2, 0, 4, 1, 0};  // the fastest 1080p shape measured on MI355X
   // device buffers (one entry per block of the batch unless noted)
   FrameDev* d_fr = nullptr;  // kMaxFrames descriptors
   FrameDev* h_fr = nullptr;  // pinned staging ring: 4 x kMaxFrames
@@ -207,7 +207,7 @@ struct mptc_ctx {
   // batch shape the maps and graphs were built for
   std::vector<int> shape;
   mptc_tuning shape_tune{0, 0, 0, 0, 0, 0, 0};
-  int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0, n_ltiles = 0;
+  int n_frames = 0, total = 0, n_wtiles = 0, n_tiles = 0, n_itiles = 0;
   GraphSet graphs;
   bool any_inter = true;  // the batch holds an inter frame
   int last_passes = 0, last_total = 0, last_intra = 0;
@@ -304,8 +304,8 @@ int check_tuning(const mptc_tuning& t) {
   if (t.pass_batch < 1 || t.pass_batch > 16) return fail(MPTC_EINVAL, "pass_batch must be in 1..16");
   if (t.win_chunks < 1 || t.win_chunks > kMaxChunks) return fail(MPTC_EINVAL, "win_chunks must be in 1..8");
   if (t.prune != 0 && t.prune != 1) return fail(MPTC_EINVAL, "prune must be 0 or 1");
-  if (t.ltile_w != 8 && t.ltile_w != 16 && t.ltile_w != 32 && t.ltile_w != 64)
-    return fail(MPTC_EINVAL, "ltile_w must be 8, 16, 32 or 64");
+  if (t.itile_w != 0 && t.itile_w != 8 && t.itile_w != 16 && t.itile_w != 32 && t.itile_w != 64)
+    return fail(MPTC_EINVAL, "itile_w must be 0, 8, 16, 32 or 64");
   return MPTC_OK;
 }
 
@@ -518,6 +518,11 @@ struct FrameIn {
   uint8_t* kind;
 };
 
+// Width of the intra fixpoint tiles: itile_w, or 8 where itile_w is 0 (the
+// plain passes then use no tiles; the learned passes always do).
+int itile_eff(const mptc_tuning& t) { return t.itile_w ? t.itile_w : 8; }
+int itile_log(const mptc_ctx* c) { return __builtin_ctz((unsigned)itile_eff(c->tune)); }
+
 // The device descriptor of one frame of a batch whose earlier frames hold
 // `base` blocks, `tb` unique tiles, `wt` wave tiles of width tw and `lt`
 // learned-pass tiles of width ltw; advances the four counts past this frame.
@@ -531,9 +536,9 @@ FrameDev frame_dev(const FrameIn& x, int tw, int ltw, int& base, int& tb, int& w
   d.tile_base = tb; d.n_tiles = (d.nb + kUniqueTile - 1) / kUniqueTile;
   d.tiles_x = (d.bw + tw - 1) / tw;
   d.wtile_base = wt;
-  d.ltile_base = lt;
-  d.ltiles_x = (d.bw + ltw - 1) / ltw;
-  lt += d.ltiles_x * ((d.bh + lth - 1) / lth);
+  d.itile_base = lt;
+  d.itiles_x = (d.bw + ltw - 1) / ltw;
+  lt += d.itiles_x * ((d.bh + lth - 1) / lth);
   base += d.nb;
   tb += d.n_tiles;
   wt += d.tiles_x * ((d.bh + th - 1) / th);
@@ -554,7 +559,7 @@ int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
       return fail(MPTC_EINVAL, "width and height must be positive multiples of 4");
     if (!x.rgb || !x.phys || !x.motion || !x.unique || !x.n_unique) return fail(MPTC_EINVAL, "null argument");
     if (!x.is_intra && !x.ref) return fail(MPTC_EINVAL, "inter frame needs the previous frame's blocks");
-    const FrameDev d = frame_dev(x, tw, c->tune.ltile_w, base, tb, wt, lt);
+    const FrameDev d = frame_dev(x, tw, itile_eff(c->tune), base, tb, wt, lt);
     if (base > c->max_total) return fail(MPTC_EINVAL, "batch holds more blocks than the context allows");
     h[f] = d;
     shape.push_back(x.w);
@@ -590,7 +595,7 @@ int prepare_batch(mptc_ctx* c, hipStream_t s, int n, const FrameIn* in) {
   c->total = base;
   c->n_wtiles = wt;
   c->n_tiles = tb;
-  c->n_ltiles = lt;
+  c->n_itiles = lt;
   return MPTC_OK;
 }
 
@@ -625,7 +630,7 @@ int infer_q(mptc_ctx* c, hipStream_t s, const NetState& n, const uint16_t* X, in
   });
   if (rc) return rc;
   return launch(c, s, KF_NET_L3, [&] {
-    auto kern = kk <= 4 ? k_net_l3<4> : kk <= 8 ? k_net_l3<8> : k_net_l3<16>;
+    auto kern = Y ? k_net_l3<4, 9> : kk <= 4 ? k_net_l3<4, 8> : kk <= 8 ? k_net_l3<8, 8> : k_net_l3<16, 8>;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kInferWaves), 0, s, h2p, rows, n.pk, Y, c->d_fr, c->d_sid, net,
                        S, kk, need, prop);
   });
@@ -657,8 +662,8 @@ int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first) {
   const int S = c->cfg.search_area, thr = c->cfg.err_threshold, kk = c->top_k;
   return launch(c, s, KF_LPASS, [&] {
     auto kern = kk <= 4 ? k_lintra_pass<4> : kk <= 8 ? k_lintra_pass<8> : k_lintra_pass<16>;
-    hipLaunchKernelGGL(kern, dim3(grid_for(c->n_ltiles * 64, 256)), dim3(256), 0, s, c->d_fr, c->n_frames,
-                       c->n_ltiles, __builtin_ctz((unsigned)c->tune.ltile_w), S, thr, kk, c->d_prop, c->d_own, c->d_need, c->d_cw, c->d_seen, c->d_errs,
+    hipLaunchKernelGGL(kern, dim3(grid_for(c->n_itiles * 64, 256)), dim3(256), 0, s, c->d_fr, c->n_frames,
+                       c->n_itiles, itile_log(c), S, thr, kk, c->d_prop, c->d_own, c->d_need, c->d_cw, c->d_seen, c->d_errs,
                        c->d_intra, c->d_stats, par, first, c->d_conv);
   });
 }
@@ -780,7 +785,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
 int passes_per_batch(const mptc_ctx* c) { return 2 * ((c->tune.pass_batch + 1) / 2); }
 
 int queue_passes(mptc_ctx* c, hipStream_t s) {
-  const int S = c->cfg.search_area, thr = c->cfg.err_threshold, total = c->total;
+  const int S = c->cfg.search_area, thr = c->cfg.err_threshold;
   const int np = passes_per_batch(c);
   int rc;
   if (c->mode == MPTC_SEARCH_LEARNED) {
@@ -791,18 +796,26 @@ int queue_passes(mptc_ctx* c, hipStream_t s) {
   }
   // the pruned search keeps passing sets beside E (prune_kernels.h)
   uint32_t* pm = use_prune(c) ? c->d_pm : nullptr;
+  // itile_w 0: re-fold passes (no skip in k_intra_reeval; skip_tile stays 0)
+  const int tw_log = itile_log(c);
   for (int p = 0; p < np; ++p) {
     const int par = p & 1;
     if ((rc = launch(c, s, KF_REEVAL, [&] {
            hipLaunchKernelGGL(k_intra_reeval, dim3(kReevalGrid), dim3(256), 0, s, c->d_fr, c->d_sid, S, thr, c->d_own,
                               c->d_need, c->d_cw, c->d_ctab, c->d_chg, c->d_E, c->nbs, c->d_dirty, c->d_orig,
-                              c->d_intra, c->d_stats, par, pm);
+                              c->d_intra, c->d_stats, par, pm, tw_log);
          })))
       return rc;
-    if ((rc = launch(c, s, KF_SELECT, [&] {
-           hipLaunchKernelGGL(k_intra_select, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total, S,
-                              thr, c->d_own, c->d_cw, c->d_ctab, c->d_E, c->nbs, c->d_dirty, c->d_intra, c->d_chg,
-                              c->d_stats, par ^ 1, c->d_conv, pm);
+    if ((rc = launch(c, s, KF_TILE, [&] {
+           if (c->tune.itile_w)
+             hipLaunchKernelGGL(k_intra_tile_pass, dim3(grid_for(c->n_itiles * 64, 256)), dim3(256), 0, s, c->d_fr,
+                                c->n_frames, c->n_itiles, tw_log, S, thr, c->d_own, c->d_need, c->d_cw, c->d_ctab,
+                                c->d_E, c->nbs, c->d_dirty, c->d_orig, c->d_intra, c->d_chg, c->d_stats, par ^ 1,
+                                c->d_conv, pm);
+           else
+             hipLaunchKernelGGL(k_intra_select, dim3(grid_for(c->total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
+                                c->total, S, thr, c->d_own, c->d_cw, c->d_ctab, c->d_E, c->nbs, c->d_dirty, c->d_intra,
+                                c->d_chg, c->d_stats, par ^ 1, c->d_conv, pm);
          })))
       return rc;
   }
@@ -1149,36 +1162,44 @@ int32_t mptc_autotune(mptc_ctx* c, int32_t n, const mptc_frame_io* frames, int32
   probe.prune = 1;
   c->tune = probe;
   const int n_prune = use_prune(c) ? 2 : 1;
-  const int ltws[] = {8, 16, 32, 64};
-  const int n_lt = c->mode == MPTC_SEARCH_LEARNED ? 4 : 1;  // the learned intra passes' tile
-  for (int lti = 0; lti < n_lt; ++lti)
-  for (int pr = 0; pr < n_prune; ++pr)
-  for (int graph = 0; graph < 2; ++graph)
-  for (int wg : wgs)
-    for (int tw : tws)
-      for (int kc : chunks) {
-        c->tune = mptc_tuning{wg, tw, keep.pass_batch, graph, kc, n_prune == 2 ? pr : keep.prune,
-                              n_lt > 1 ? ltws[lti] : keep.ltile_w};
-        int rc = mptc_encode_batch_device(c, n, frames, nullptr);  // warm-up: maps and graphs
-        if (rc) {
-          c->tune = keep;
-          return rc;
-        }
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        const auto t0 = std::chrono::steady_clock::now();
-        for (int i = 0; i < steps; ++i)
-          if ((rc = mptc_encode_batch_device(c, n, frames, nullptr))) {
-            c->tune = keep;
-            return rc;
-          }
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        const double ms =
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
-        if (ms < win_ms) {
-          win_ms = ms;
-          win = c->tune;
-        }
-      }
+  // times the shape t (warm-up: maps and graphs; then `steps` encodes) and keeps the fastest
+  auto trial = [&](const mptc_tuning& t) -> int {
+    c->tune = t;
+    int rc = mptc_encode_batch_device(c, n, frames, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < steps; ++i)
+      if ((rc = mptc_encode_batch_device(c, n, frames, nullptr))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+    if (ms < win_ms) {
+      win_ms = ms;
+      win = t;
+    }
+    return MPTC_OK;
+  };
+  int rc = MPTC_OK;
+  for (int pr = 0; pr < n_prune && !rc; ++pr)
+    for (int graph = 0; graph < 2 && !rc; ++graph)
+      for (int wg : wgs)
+        for (int tw : tws)
+          for (int kc : chunks)
+            if (!rc)
+              rc = trial(mptc_tuning{wg, tw, keep.pass_batch, graph, kc, n_prune == 2 ? pr : keep.prune,
+                                     keep.itile_w});
+  // then the intra fixpoint tile on the winning shape
+  const mptc_tuning shape = win;
+  for (int itw : {0, 8, 16, 32, 64})
+    if (!rc && itw != shape.itile_w) {
+      mptc_tuning t = shape;
+      t.itile_w = itw;
+      rc = trial(t);
+    }
+  if (rc) {
+    c->tune = keep;
+    return rc;
+  }
   c->tune = win;
   if (best) *best = win;
   if (best_ms) *best_ms = win_ms;
@@ -1690,7 +1711,7 @@ int na_setup(int device, NoAllocDev& nd) {
   c.d_pm = static_cast<uint32_t*>(at(NA_PM));
   c.d_conv = static_cast<uint32_t*>(at(NA_CONV));
   c.h_conv = nd.h_conv;
-  c.tune = mptc_tuning{256, 16, 2, 0, 4, 1, 8};  // direct launches
+  c.tune = mptc_tuning{256, 16, 2, 0, 4, 1, 0};  // direct launches
   int cu = 0;
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c.n_cu = cu;
   nd.ready = true;
@@ -1732,7 +1753,7 @@ int32_t mptc_encode_frame_noalloc(int32_t device, const uint8_t* d_rgb, int32_t 
   const FrameIn in{d_rgb,          w, h, is_intra, is_intra ? nullptr : d_ref_phys, d_phys_out, d_motion_out,
                    d_unique_out, d_n_unique_out, d_eps_out, d_kind_out};
   int base = 0, tb = 0, wt = 0, lt = 0;
-  nd.h_fr = frame_dev(in, c->tune.tile_w, c->tune.ltile_w, base, tb, wt, lt);
+  nd.h_fr = frame_dev(in, c->tune.tile_w, itile_eff(c->tune), base, tb, wt, lt);
   HIP_TRY(hipMemcpyAsync(c->d_fr, &nd.h_fr, sizeof(FrameDev), hipMemcpyHostToDevice, s));
   // one frame: every block and wave tile belongs to frame 0; the pass count
   // word cleared (a step whose first search changes nothing has no pass that
@@ -1746,7 +1767,7 @@ int32_t mptc_encode_frame_noalloc(int32_t device, const uint8_t* d_rgb, int32_t 
   c->total = base;
   c->n_wtiles = wt;
   c->n_tiles = tb;
-  c->n_ltiles = lt;
+  c->n_itiles = lt;
   c->any_inter = !is_intra;
   c->last_intra = is_intra;
   if ((rc = queue_prefix(c, s, true))) return rc;

@@ -273,22 +273,24 @@ __device__ __forceinline__ void key_insert(uint32_t (&t)[K], uint32_t key) {
   for (int i = 0; i < K; ++i) t[i] = n[i];
 }
 
-// Layer 3.  With Y non-null: writes the 262 outputs of each row (row-major).
-// Otherwise: for each block g of the batch (rows = total) that `need`
+// Layer 3.  NT = 9: writes the 262 outputs of each row to Y (row-major).
+// NT = 8 (the 256 logit columns only; the endpoint-seed outputs are not
+// computed): for each block g of the batch (rows = total) that `need`
 // admits (need null: every block of an inter frame), the top kk <= K
 // admitted offsets by score go to prop[g * kk + i] (-1 past the last).
-template <int K>
+template <int K, int NT>
 __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __restrict__ H2p, int rows,
                                                              const uint16_t* __restrict__ pk, float* __restrict__ Y,
                                                              const FrameDev* __restrict__ fr,
                                                              const uint8_t* __restrict__ sid, int net, int S, int kk,
                                                              const uint8_t* __restrict__ need,
                                                              int16_t* __restrict__ prop) {
-  __shared__ uint4 sW3[kL3Rows * kNetHid / 8];  // 144 KiB, rows of 32 chunks, swizzled
-  __shared__ __attribute__((aligned(16))) float sb[kL3Rows];
-  stage_rows(sW3, reinterpret_cast<const uint4*>(pk + kPkW3), kL3Rows, kNetHid / 8, true);
+  static_assert(NT == 8 || NT == 9, "8 logit tiles, or all 9 output tiles");
+  __shared__ uint4 sW3[NT * 32 * kNetHid / 8];  // 128 or 144 KiB, rows of 32 chunks, swizzled
+  __shared__ __attribute__((aligned(16))) float sb[NT * 32];
+  stage_rows(sW3, reinterpret_cast<const uint4*>(pk + kPkW3), NT * 32, kNetHid / 8, true);
   const float* bias = reinterpret_cast<const float*>(pk + kPkBias) + 2 * kNetHid;
-  for (int i = threadIdx.x; i < kL3Rows; i += blockDim.x) sb[i] = bias[i];
+  for (int i = threadIdx.x; i < NT * 32; i += blockDim.x) sb[i] = bias[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, m = lane & 31, h = lane >> 5;
   const int n_tiles = (rows + 31) / 32;
@@ -296,24 +298,24 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
   for (int tile = wave_g; tile < n_tiles; tile += gridDim.x * kInferWaves) {
     const int row = tile * 32 + m;
     // accumulators start at the biases of the lane's outputs
-    f32x16_t acc[9];
+    f32x16_t acc[NT];
 #pragma unroll
-    for (int nb = 0; nb < 9; ++nb) acc[nb] = bias_tile(sb, nb, h);
+    for (int nb = 0; nb < NT; ++nb) acc[nb] = bias_tile(sb, nb, h);
     const uint4* in = H2p + (size_t)tile * (kH2TileBytes / 16);
 #pragma unroll 2
     for (int s = 0; s < 16; ++s) {
       const bf16x8_t b = as_bf16x8(in[s * 64 + lane]);
 #pragma unroll
-      for (int nb = 0; nb < 9; ++nb) {
+      for (int nb = 0; nb < NT; ++nb) {
         const int r = nb * 32 + m;
         const bf16x8_t a = as_bf16x8(sW3[r * 32 + ((2 * s + h) ^ (r & kSwz))]);
         acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nb], 0, 0, 0);
       }
     }
-    if (Y) {
+    if (NT == 9) {
       if (row < rows) {
 #pragma unroll
-        for (int nb = 0; nb < 9; ++nb)
+        for (int nb = 0; nb < NT; ++nb)
 #pragma unroll
           for (int i = 0; i < 16; ++i) {
             const int u = nb * 32 + 8 * (i >> 2) + 4 * h + (i & 3);

@@ -34,7 +34,7 @@ class Config(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("wg_size", C.c_int32), ("tile_w", C.c_int32), ("pass_batch", C.c_int32),
                 ("use_graph", C.c_int32), ("win_chunks", C.c_int32), ("prune", C.c_int32),
-                ("ltile_w", C.c_int32)]
+                ("itile_w", C.c_int32)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -66,6 +66,19 @@ class TrainCfg(C.Structure):
 
 NET_INTER, NET_INTRA = 0, 1
 NET_IN, NET_HIDDEN, NET_OUT, NET_LOGITS, NET_PARAMS = 48, 256, 262, 256, 145670
+# offsets in the flat parameter vector: W1 [48][256], b1, W2 [256][256], b2, W3 [256][262], b3
+NET_OFF_W3 = NET_IN * NET_HIDDEN + NET_HIDDEN + NET_HIDDEN * NET_HIDDEN + NET_HIDDEN
+NET_OFF_B3 = NET_OFF_W3 + NET_HIDDEN * NET_OUT
+assert NET_OFF_B3 + NET_OUT == NET_PARAMS
+
+
+def zero_logit_layer(params: np.ndarray) -> np.ndarray:
+    """A copy of a parameter vector with the logit columns of W3 and the logit
+    biases of b3 at zero (the network's logits then equal its base scores)."""
+    p = np.array(params, np.float32, copy=True)
+    p[NET_OFF_W3:NET_OFF_B3].reshape(NET_HIDDEN, NET_OUT)[:, :NET_LOGITS] = 0.0
+    p[NET_OFF_B3:NET_OFF_B3 + NET_LOGITS] = 0.0
+    return p
 SEARCH_EXHAUSTIVE, SEARCH_LEARNED_INTER, SEARCH_LEARNED = 0, 1, 2
 
 _u8p = C.POINTER(C.c_uint8)
@@ -249,8 +262,8 @@ class Encoder:
         _check(self.L.mptc_get_tuning(self.h, C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, wg_size=256, tile_w=16, pass_batch=2, use_graph=0, win_chunks=4, prune=1, ltile_w=8):
-        t = Tuning(wg_size, tile_w, pass_batch, use_graph, win_chunks, prune, ltile_w)
+    def set_tuning(self, wg_size=256, tile_w=16, pass_batch=2, use_graph=0, win_chunks=4, prune=1, itile_w=0):
+        t = Tuning(wg_size, tile_w, pass_batch, use_graph, win_chunks, prune, itile_w)
         _check(self.L.mptc_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, frames: list[dict], steps: int = 3):

@@ -80,11 +80,11 @@ def test_batch_of_uneven_streams_matches_reference(orc):
     enc.close()
 
 
-@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune", [(256, 64, 2, 1, 2, 1), (64, 8, 1, 0, 1, 1),
-                                                         (128, 16, 2, 1, 4, 0), (256, 32, 3, 0, 8, 1),
-                                                         (64, 64, 16, 1, 6, 0), (256, 16, 1, 1, 3, 1),
-                                                         (256, 8, 2, 0, 4, 1)])
-def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks, prune):
+@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune,itw", [(256, 64, 2, 1, 2, 1, 8), (64, 8, 1, 0, 1, 1, 0),
+                                                             (128, 16, 2, 1, 4, 0, 32), (256, 32, 3, 0, 8, 1, 64),
+                                                             (64, 64, 16, 1, 6, 0, 16), (256, 16, 1, 1, 3, 1, 32),
+                                                             (256, 8, 2, 0, 4, 1, 0), (256, 16, 2, 0, 4, 0, 64)])
+def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks, prune, itw):
     from mptc_amd import hip, lib
     from oracle import SynthCfg
     sizes = [(160, 96, 11), (96, 64, 12), (256, 128, 13)]
@@ -93,9 +93,9 @@ def test_tunings_are_bit_identical(orc, wg, tw, pb, graph, chunks, prune):
     want = [orc.encode_sequence(fr, S, thr, 3) for fr in frames]
     total = sum((w // 4) * (h // 4) for w, h, _ in sizes)
     enc = lib.Encoder(256, 128, S, thr, max_batch_frames=3, max_batch_blocks=total)
-    enc.set_tuning(wg, tw, pb, graph, chunks, prune)
+    enc.set_tuning(wg, tw, pb, graph, chunks, prune, itw)
     streams = [Stream(hip, fr) for fr in frames]
-    log(f"tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune}")
+    log(f"tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune} itile_w={itw}")
     res = _encode_streams(enc, streams, 4, 3)
     st = enc.stats()
     exact = st["exact_inter_candidates"] + st["exact_intra_candidates"]
@@ -120,6 +120,7 @@ def test_autotune_picks_a_valid_shape(orc):
     best, ms = enc.autotune([st.io(1, False)], steps=2)
     log(f"autotune best {best} {ms:.3f} ms/step")
     assert best["wg_size"] in (64, 128, 256) and best["tile_w"] in (64, 32, 16, 8) and ms > 0
+    assert best["itile_w"] in (0, 8, 16, 32, 64)
     assert enc.get_tuning() == best
     # the tuned shape still encodes bit-exactly
     enc.encode_batch_device([st.io(0, True)])
@@ -129,9 +130,9 @@ def test_autotune_picks_a_valid_shape(orc):
     enc.close()
 
 
-@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune", [(256, 8, 2, 0, 4, 1), (64, 64, 1, 1, 1, 0),
-                                                         (128, 32, 3, 1, 8, 1)])
-def test_tunings_at_1080p_match_reference(orc, wg, tw, pb, graph, chunks, prune):
+@pytest.mark.parametrize("wg,tw,pb,graph,chunks,prune,itw", [(256, 8, 2, 0, 4, 1, 0), (64, 64, 1, 1, 1, 0, 64),
+                                                             (128, 32, 3, 1, 8, 1, 16), (256, 16, 2, 0, 2, 1, 32)])
+def test_tunings_at_1080p_match_reference(orc, wg, tw, pb, graph, chunks, prune, itw):
     """Launch shapes at the 1080p size the bench times (the first one is the
     shape the bench's autotuner picks there), against the reference's fixture."""
     from mptc_amd import hip, lib
@@ -141,9 +142,9 @@ def test_tunings_at_1080p_match_reference(orc, wg, tw, pb, graph, chunks, prune)
     n = 4
     frames = orc.synth_frames(synth_cfg(spec), n)
     enc = lib.Encoder(spec["width"], spec["height"], spec["S"], spec["thr"])
-    enc.set_tuning(wg, tw, pb, graph, chunks, prune)
+    enc.set_tuning(wg, tw, pb, graph, chunks, prune, itw)
     st = Stream(hip, frames)
-    log(f"1080p tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune}")
+    log(f"1080p tuning wg={wg} tile_w={tw} passes={pb} graph={graph} chunks={chunks} prune={prune} itile_w={itw}")
     res = _encode_streams(enc, [st], n, spec["intra_interval"])
     for f in range(n):
         compare_frame(res[f][0], fx["frames"][f], f"{name} frame {f}")

@@ -472,22 +472,22 @@ def test_learned_intra_tile_shapes_agree(env, orc, name, thr):
     frames = lib.synth_frames(synth_cfg(spec), 0, 4)
     S = spec["S"]
     outs = {}
-    for ltw in (8, 16, 32, 64):
-        log(f"learned intra tile {ltw} x {64 // ltw}, {name} thr={thr}")
+    for ltw in (8, 16, 32, 64, 0):
+        log(f"learned intra tile {ltw} x {64 // max(ltw, 8)}, {name} thr={thr}")
         seq = Seq(hip, frames)
         enc = lib.Encoder(seq.w, seq.h, S, thr)
         enc.set_prior(0, nearest_prior(0))
         enc.set_prior(1, nearest_prior(1))
         enc.set_search_mode(2, 8)
-        enc.set_tuning(ltile_w=ltw)
-        assert enc.get_tuning()["ltile_w"] == ltw
+        enc.set_tuning(itile_w=ltw)
+        assert enc.get_tuning()["itile_w"] == ltw
         stats = encode(enc, lib, seq, 3, check=(orc, S, thr) if ltw == 32 else None)
         outs[ltw] = ([seq.phys[f].download(np.uint64, seq.nb) for f in range(seq.n)],
                      [seq.eps[f].download(np.uint8, 6 * seq.nb) for f in range(seq.n)],
                      [seq.motion[f].download(np.uint8, 2 * seq.nb) for f in range(seq.n)],
                      [(s["n_inter"], s["n_intra"], s["n_unique"]) for s in stats])
         enc.close()
-    for ltw in (16, 32, 64):
+    for ltw in (16, 32, 64, 0):
         for f in range(len(frames)):
             for part in range(3):
                 assert np.array_equal(outs[8][part][f], outs[ltw][part][f]), (ltw, f, part)
@@ -495,6 +495,6 @@ def test_learned_intra_tile_shapes_agree(env, orc, name, thr):
     with pytest.raises(lib.MptcError):
         e = lib.Encoder(64, 64, 8, 0)
         try:
-            e.set_tuning(ltile_w=12)
+            e.set_tuning(itile_w=12)
         finally:
             e.close()
