@@ -488,7 +488,10 @@ __device__ __forceinline__ Fold fold_passing(const uint2* __restrict__ E4, size_
 // Passing sets: for windows of at most 256 positions (S <= 8), bit p % 32 of
 // word p / 32 of pm[8 g .. 8 g + 7] is set exactly when block g's delta at
 // position p passes (valid, <= thr).  Folding a block then reads its 32 bytes
-// of set and the few passing deltas instead of all of its E.
+// of set and the few passing deltas instead of all of its E.  Where a block
+// keeps a passing set, E holds only its passing deltas (the pruned search
+// writes only groups of four with a passing one, the passes only passing
+// values); an entry whose bit is clear is not read.
 constexpr int kPassWords = 8;
 
 __device__ __forceinline__ Fold fold_passing_set(const uint32_t* __restrict__ pm, const uint2* __restrict__ E4,
@@ -677,9 +680,18 @@ __device__ __forceinline__ unsigned long long reeval_items(
     const uint64_t dc = dec[k];
     const bool is_sel =
         dec_acc(dc) && (W2 - 1 - (int)((dc >> 40) & 0xFF)) * W2 + (W2 - 1 - (int)((dc >> 32) & 0xFF)) == s;
-    const int16_t old = *slot;
-    if (old != ev) *slot = ev;
-    const bool passes_old = old != kNoCand && old <= thr, passes_new = ev != kNoCand && ev <= thr;
+    const bool passes_new = ev != kNoCand && ev <= thr;
+    bool passes_old;
+    int16_t old;
+    if (pm) {  // E is kept only where the passing set has the position
+      passes_old = (pm[(size_t)k * kPassWords + (s >> 5)] >> (s & 31)) & 1u;
+      old = passes_old ? *slot : kNoCand;
+      if (passes_new && old != ev) *slot = ev;
+    } else {
+      old = *slot;
+      passes_old = old != kNoCand && old <= thr;
+      if (old != ev) *slot = ev;
+    }
     if ((old != ev && (passes_old || passes_new)) || is_sel) dirty[k] = 1u;
     // one work item per (block, position) in a pass: bits of a word flip
     // concurrently only for different positions
@@ -904,11 +916,20 @@ __global__ void __launch_bounds__(256) k_intra_tile_pass(
         const int s = r * W2 + t;
         int16_t* slot = &E[((size_t)(s >> 2) * nbs + g) * 4 + (s & 3)];
         ++n_eval;
-        const int16_t old = *slot;
-        if (old != ev) *slot = ev;
+        const bool passes_new = ev != kNoCand && ev <= thr;
+        bool passes_old;
+        int16_t old;
+        if (pm) {  // E is kept only where the passing set has the position
+          passes_old = (pm[(size_t)g * kPassWords + (s >> 5)] >> (s & 31)) & 1u;
+          old = passes_old ? *slot : kNoCand;
+          if (passes_new && old != ev) *slot = ev;
+        } else {
+          old = *slot;
+          passes_old = old != kNoCand && old <= thr;
+          if (old != ev) *slot = ev;
+        }
         const bool is_sel = dec_acc(dc) && (W2 - 1 - (int)((dc >> 40) & 0xFF)) * W2 +
                                                    (W2 - 1 - (int)((dc >> 32) & 0xFF)) == s;
-        const bool passes_old = old != kNoCand && old <= thr, passes_new = ev != kNoCand && ev <= thr;
         if ((old != ev && (passes_old || passes_new)) || is_sel) redo = true;
         if (pm && passes_old != passes_new) pm[(size_t)g * kPassWords + (s >> 5)] ^= 1u << (s & 31);
       }

@@ -298,12 +298,14 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
       for (int k = 0; k < nb; k += 4) {
         const int16_t e0 = s_e[wave][k][lane], e1 = s_e[wave][k + 1][lane], e2 = s_e[wave][k + 2][lane],
                       e3 = s_e[wave][k + 3][lane];
-        const uint32_t lo = (uint32_t)(uint16_t)e0 | ((uint32_t)(uint16_t)e1 << 16);
-        const uint32_t hi = (uint32_t)(uint16_t)e2 | ((uint32_t)(uint16_t)e3 << 16);
-        E4[(size_t)((pb + k) >> 2) * nbs + g] = make_uint2(lo, hi);
-        bits |= (uint32_t)((e0 != kNoCand && e0 <= thr) | (e1 != kNoCand && e1 <= thr) << 1 |
-                           (e2 != kNoCand && e2 <= thr) << 2 | (e3 != kNoCand && e3 <= thr) << 3)
-                << k;
+        const uint32_t gb = (uint32_t)((e0 != kNoCand && e0 <= thr) | (e1 != kNoCand && e1 <= thr) << 1 |
+                                       (e2 != kNoCand && e2 <= thr) << 2 | (e3 != kNoCand && e3 <= thr) << 3);
+        if (gb) {  // E is kept only where a delta passes (the passing set says where)
+          const uint32_t lo = (uint32_t)(uint16_t)e0 | ((uint32_t)(uint16_t)e1 << 16);
+          const uint32_t hi = (uint32_t)(uint16_t)e2 | ((uint32_t)(uint16_t)e3 << 16);
+          E4[(size_t)((pb + k) >> 2) * nbs + g] = make_uint2(lo, hi);
+        }
+        bits |= gb << k;
       }
       if (bits) {  // rare; the batch's 16 bits may straddle two words, which other chunks' waves share
         const int sh = pb & 31;
