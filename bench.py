@@ -179,8 +179,8 @@ def profiled_traffic(kernel):
         return None
     for line in open(TRAFFIC_PROFILE):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == sym:
-            return round((float(cells[3]) + float(cells[4])) * 1e6)
+        if len(cells) >= 5 and cells[0] == sym:  # the last two columns: HBM read and write MB
+            return round((float(cells[-2]) + float(cells[-1])) * 1e6)
     return None
 
 
@@ -540,6 +540,21 @@ def measure_config4(hip, lib, steps, warmup):
     run(1)
     stream.synchronize()
     tuning, _ = enc.autotune(ios(1, False), steps=2)
+    # the autotuner times one inter step; the fixpoint tile is then chosen on
+    # the timed mix (an intra step first), where the intra frames' passes weigh
+    best = None
+    for itw in (0, 8, 64):
+        enc.set_tuning(**dict(tuning, itile_w=itw))
+        run(1)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        run(steps)
+        stream.synchronize()
+        dt = time.perf_counter() - t0
+        if best is None or dt < best[0]:
+            best = (dt, itw)
+    tuning = dict(tuning, itile_w=best[1])
+    enc.set_tuning(**tuning)
     run(warmup)
     stream.synchronize()
     hip.synchronize()
@@ -765,7 +780,7 @@ def main():
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(elapsed / a.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32",
         "data": "synthetic (seeded generator, mptc_amd/csrc/framegen.h)",
-        "config": {"workload": f"1080p encode ({a.workload}): {spec['frames']} frames {W}x{H} RGB8, "
+        "config": {"workload": f"encode ({a.workload}): {spec['frames']} frames {W}x{H} RGB8, "
                                f"S={spec['S']} ({2 * spec['S']}-block window), threshold {spec['thr']}, "
                                f"intra every {spec['intra_interval']} frames; value = plain search, "
                                f"bit-exact to the reference", "blocks_per_frame": nb,
