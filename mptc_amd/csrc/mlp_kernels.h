@@ -460,7 +460,8 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
                                                      uint32_t* __restrict__ cw, uint32_t* __restrict__ seen,
                                                      int* __restrict__ errs, uint64_t* __restrict__ dec,
                                                      DevStats* __restrict__ st, int par, int first,
-                                                     unsigned int* __restrict__ conv) {
+                                                     unsigned int* __restrict__ conv, uint8_t* __restrict__ tflag,
+                                                     long tf_stride) {
   __shared__ uint8_t s_snap[512];
   __shared__ float2 s_ab[4];
   __shared__ unsigned long long s_acc;
@@ -480,7 +481,18 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
   const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int lane = (int)(threadIdx.x & 63);
   unsigned long long n_cand = 0, n_re = 0;  // first evaluations, re-evaluations
+  // Tiles to do: every tile in the first pass; later, the tiles the previous
+  // pass flagged (a word some block of the tile may propose changed, or the
+  // tile did not settle).  The other tiles' evaluations are all current.  A
+  // pass reads and clears the flags of parity par ^ 1 and sets those of
+  // parity par; stale set flags only cost a redo.
+  bool todo = false;
   if (wv < n_itiles) {
+    uint8_t* fl = tflag + (size_t)(par ^ 1) * tf_stride + wv;
+    todo = first || *fl != 0;
+    if (lane == 0) *fl = 0;
+  }
+  if (todo) {
     int f = 0;
     while (f + 1 < n_frames && fr[f + 1].itile_base <= wv) ++f;
     const FrameDev& d = fr[f];
@@ -574,6 +586,7 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
         }
       }
     }
+    bool settled = false;
     for (int round = 0; round < kMaxRounds; ++round) {
       // current words of the proposals inside the tile
 #pragma unroll
@@ -607,7 +620,10 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
       // a lane publishes once nothing of it is stale (first pass: after its
       // first full set of evaluations even if nothing changed)
       const bool publish = act && j < 0 && !clean_once;
-      if (__ballot(dirty || publish) == 0ull) break;
+      if (__ballot(dirty || publish) == 0ull) {
+        settled = true;
+        break;
+      }
       if (dirty) {
         if (!have_be) {
           BlockPx px;
@@ -641,6 +657,7 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
       }
     }
     if (act && !clean_once) decide();  // round limit reached: store a consistent state
+    if (!settled && lane == 0) tflag[(size_t)par * tf_stride + wv] = 1;  // redo what is stale next pass
     if (published) {
 #pragma unroll
       for (int i = 0; i < K; ++i)
@@ -652,6 +669,13 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
       if (cur != start) {
         cw[g] = cur;
         atomicAdd(&s_chg, 1u);
+        // the tiles of the blocks that may propose this one: rows by .. by + 15,
+        // columns bx - 7 .. bx + 8 (offsets dx in -8..7, dy in 0..15)
+        const int c0 = max(bx - 7, 0) >> ltw_log, c1 = min(bx + 8, d.bw - 1) >> ltw_log;
+        const int r0 = by >> lth_log, r1 = min(by + 15, d.bh - 1) >> lth_log;
+        uint8_t* fl = tflag + (size_t)par * tf_stride + d.itile_base;
+        for (int ty2 = r0; ty2 <= r1; ++ty2)
+          for (int tx2 = c0; tx2 <= c1; ++tx2) fl[ty2 * d.itiles_x + tx2] = 1;
       }
     }
   }

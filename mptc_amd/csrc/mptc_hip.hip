@@ -116,8 +116,8 @@ const char* kKernelNames[KF_N] = {"fit",          "inter",
                                   "pruned_inter", "pruned_intra", "fold_intra"};
 constexpr int kMaxChunks = 8;  // window-row chunks per block in the search kernels
 
-constexpr int kReevalGrid = 4096;  // workgroups of the re-evaluation pass (grid-stride)
-constexpr int kListGrid = 256;     // workgroups of the re-select pass (grid-stride)
+constexpr int kReevalGrid = 1024;  // workgroups of the re-evaluation pass (grid-stride; 1024 measured faster than 4096)
+constexpr int kListGrid = 256;     // workgroups of the list-apply pass (grid-stride)
 
 struct Graphs {
   hipGraphExec_t prefix = nullptr, passes = nullptr, suffix = nullptr;
@@ -228,6 +228,7 @@ struct mptc_ctx {
   uint16_t* d_amask[2] = {nullptr, nullptr};  // inter, intra: 16 rows of 16 offset bits per block
   uint32_t* d_seen = nullptr;
   int* d_errs = nullptr;
+  uint8_t* d_tflag = nullptr;  // learned intra passes: tiles to redo, by pass parity (2 x max_total)
   // profiling
   bool prof = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -664,7 +665,7 @@ int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first) {
     auto kern = kk <= 4 ? k_lintra_pass<4> : kk <= 8 ? k_lintra_pass<8> : k_lintra_pass<16>;
     hipLaunchKernelGGL(kern, dim3(grid_for(c->n_itiles * 64, 256)), dim3(256), 0, s, c->d_fr, c->n_frames,
                        c->n_itiles, itile_log(c), S, thr, kk, c->d_prop, c->d_own, c->d_need, c->d_cw, c->d_seen, c->d_errs,
-                       c->d_intra, c->d_stats, par, first, c->d_conv);
+                       c->d_intra, c->d_stats, par, first, c->d_conv, c->d_tflag, (long)c->max_total);
   });
 }
 
@@ -1035,7 +1036,7 @@ void mptc_destroy(mptc_ctx* c) {
   c->nw.release();
   c->ew.release();
   void* lbufs[] = {c->d_h2p, c->d_pinter, c->d_prop, c->d_seen, c->d_errs, c->d_amask[0], c->d_amask[1],
-                   c->d_rbt,  c->d_cbt,  c->d_pm};
+                   c->d_rbt,  c->d_cbt,  c->d_pm,  c->d_tflag};
   for (void* p : lbufs)
     if (p) (void)hipFree(p);
   if (c->h_stats) (void)hipHostFree(c->h_stats);
@@ -1487,6 +1488,10 @@ int32_t mptc_set_search_mode(mptc_ctx* c, int32_t mode, int32_t top_k) {
                       (rc = dalloc(&c->d_pinter, nb * kMaxTopK)) || (rc = dalloc(&c->d_prop, nb * kMaxTopK)) ||
                       (rc = dalloc(&c->d_seen, nb * kMaxTopK)) || (rc = dalloc(&c->d_errs, nb * kMaxTopK))))
       return rc;
+    if (!c->d_tflag) {  // one flag per tile and parity; a tile holds at least one block
+      if ((rc = dalloc(&c->d_tflag, 2 * nb))) return rc;
+      HIP_TRY(hipMemset(c->d_tflag, 0, 2 * nb));
+    }
   }
   if (mode != c->mode || top_k != c->top_k) c->graphs.destroy();
   c->mode = mode;
