@@ -147,6 +147,15 @@ __device__ __forceinline__ void pass_end(DevStats* st, int par, unsigned int* cl
   }
 }
 
+// An empty pass (the previous one changed no word) that reports to the host
+// (conv non-null: the last pass of a batch) writes the count 0.
+__device__ __forceinline__ void report_fixpoint(unsigned int* conv) {
+  if (conv && blockIdx.x == 0 && threadIdx.x == 0) {
+    *reinterpret_cast<volatile unsigned int*>(conv) = 0u;
+    __threadfence_system();
+  }
+}
+
 // Appends v (where want) to a global list with one atomic per workgroup.
 // Every thread of the workgroup must call it; s_base is a __shared__ word.
 __device__ __forceinline__ void wg_append(bool want, int v, int* list, unsigned int* count, unsigned int* s_base) {
@@ -783,7 +792,10 @@ __global__ void __launch_bounds__(256) k_intra_select(const FrameDev* __restrict
   // No workgroup of this pass writes that counter, so either every workgroup
   // returns here or none does; an empty pass reports nothing (pass_end): the
   // pass that produced the empty list already wrote 0 to *conv.
-  if (st->n_changed[par ^ 1] == 0) return;
+  if (st->n_changed[par ^ 1] == 0) {
+    report_fixpoint(conv);
+    return;
+  }
   __shared__ unsigned int s_base;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool changed = k < total && select_block(fr, sid, k, S, thr, own, cw, ctab, E4, nbs, dirty, dec, pm);
@@ -818,7 +830,10 @@ __global__ void __launch_bounds__(256) k_intra_tile_pass(
   // No workgroup of this pass writes that counter, so either every workgroup
   // returns here or none does; an empty pass reports nothing (pass_end): the
   // pass that produced the empty list already wrote 0 to *conv.
-  if (st->n_changed[par ^ 1] == 0) return;
+  if (st->n_changed[par ^ 1] == 0) {
+    report_fixpoint(conv);
+    return;
+  }
   __shared__ uint8_t s_snap[512];
   __shared__ float2 s_ab[4];
   __shared__ unsigned long long s_acc;

@@ -474,7 +474,10 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
   // stays zero and the previous pass already wrote 0 to *conv.
   if (threadIdx.x == 0) s_prev = first ? 1u : st->n_changed[par ^ 1];
   __syncthreads();
-  if (s_prev == 0) return;
+  if (s_prev == 0) {
+    report_fixpoint(conv);
+    return;
+  }
   load_lds_tables(s_snap, s_ab);
   if (threadIdx.x == 0) s_chg = 0;
   __syncthreads();

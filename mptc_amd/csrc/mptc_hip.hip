@@ -659,13 +659,16 @@ int proposals_q(mptc_ctx* c, hipStream_t s, int net, const uint8_t* need, int16_
   return infer_q(c, s, n, c->ew.X, total, c->d_h2p, nullptr, net, c->top_k, need, prop);
 }
 
-int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first) {
+// report: the pass writes its count of changed words for the host (the last
+// pass of a batch); the others do not.
+int lintra_pass_q(mptc_ctx* c, hipStream_t s, int par, int first, bool report) {
   const int S = c->cfg.search_area, thr = c->cfg.err_threshold, kk = c->top_k;
   return launch(c, s, KF_LPASS, [&] {
     auto kern = kk <= 4 ? k_lintra_pass<4> : kk <= 8 ? k_lintra_pass<8> : k_lintra_pass<16>;
     hipLaunchKernelGGL(kern, dim3(grid_for(c->n_itiles * 64, 256)), dim3(256), 0, s, c->d_fr, c->n_frames,
                        c->n_itiles, itile_log(c), S, thr, kk, c->d_prop, c->d_own, c->d_need, c->d_cw, c->d_seen, c->d_errs,
-                       c->d_intra, c->d_stats, par, first, c->d_conv, c->d_tflag, (long)c->max_total);
+                       c->d_intra, c->d_stats, par, first, report ? c->d_conv : nullptr, c->d_tflag,
+                       (long)c->max_total);
   });
 }
 
@@ -743,7 +746,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
     return rc;
   if (c->mode == MPTC_SEARCH_LEARNED) {
     if ((rc = proposals_q(c, s, MPTC_NET_INTRA, c->d_need, c->d_prop, x_done))) return rc;
-    return lintra_pass_q(c, s, 0, 1);
+    return lintra_pass_q(c, s, 0, 1, false);
   }
   if (prune) {  // bound tables and cleared passing sets from k_intra_prep
     if ((rc = launch(c, s, KF_PRUNED_INTRA, [&] {
@@ -792,7 +795,7 @@ int queue_passes(mptc_ctx* c, hipStream_t s) {
   if (c->mode == MPTC_SEARCH_LEARNED) {
     // parities 1, 0, ..., 0: the batch's last pass counts into n_changed[0]
     for (int p = 0; p < np; ++p)
-      if ((rc = lintra_pass_q(c, s, (p + 1) & 1, 0))) return rc;
+      if ((rc = lintra_pass_q(c, s, (p + 1) & 1, 0, p == np - 1))) return rc;
     return MPTC_OK;
   }
   // the pruned search keeps passing sets beside E (prune_kernels.h)
@@ -812,11 +815,11 @@ int queue_passes(mptc_ctx* c, hipStream_t s) {
              hipLaunchKernelGGL(k_intra_tile_pass, dim3(grid_for(c->n_itiles * 64, 256)), dim3(256), 0, s, c->d_fr,
                                 c->n_frames, c->n_itiles, tw_log, S, thr, c->d_own, c->d_need, c->d_cw, c->d_ctab,
                                 c->d_E, c->nbs, c->d_dirty, c->d_orig, c->d_intra, c->d_chg, c->d_stats, par ^ 1,
-                                c->d_conv, pm);
+                                p == np - 1 ? c->d_conv : nullptr, pm);
            else
              hipLaunchKernelGGL(k_intra_select, dim3(grid_for(c->total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
                                 c->total, S, thr, c->d_own, c->d_cw, c->d_ctab, c->d_E, c->nbs, c->d_dirty, c->d_intra,
-                                c->d_chg, c->d_stats, par ^ 1, c->d_conv, pm);
+                                c->d_chg, c->d_stats, par ^ 1, p == np - 1 ? c->d_conv : nullptr, pm);
          })))
       return rc;
   }
@@ -865,10 +868,10 @@ int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
   const bool graphs = c->tune.use_graph && !c->prof && fit;
   Graphs& gr = c->graphs.v[c->any_inter ? 1 : 0];
   int rc;
-  // Passes that change words write their count to h_conv; a step whose first
-  // search changed nothing has no such pass (its passes return at once).  No
-  // pass of an earlier step can still write it: they ended at a zero count,
-  // and the passes queued after that one return at once.
+  // The last pass of each batch writes its count of changed words to h_conv
+  // (0 when it had nothing to do).  Passes of an earlier step that are still
+  // queued can only write 0 (they follow a pass that changed nothing), and
+  // they run before this step's passes on the ordered stream.
   *reinterpret_cast<volatile uint32_t*>(c->h_conv) = 0u;
   if (graphs && !gr.prefix) {
     if ((rc = capture(s, &gr.prefix, [&] { return queue_prefix(c, s, true); }))) return rc;
@@ -908,10 +911,11 @@ int run_step(mptc_ctx* c, hipStream_t s, bool fit) {
     while ((q = hipEventQuery(c->batch_ev[slot])) == hipErrorNotReady) {
     }
     if (q != hipSuccess) return fail(MPTC_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
-    // Every pass ends by writing its count of changed words to h_conv, so once
-    // the batch is done h_conv holds the count of its last pass or of a later
-    // one.  A pass that changes nothing proves the fixpoint (every later pass
-    // is empty too); the batch queued after it then runs empty.
+    // The last pass of every batch writes its count of changed words to
+    // h_conv, so once the batch is done h_conv holds the count of its last
+    // pass or of a later batch's last pass.  A pass that changes nothing
+    // proves the fixpoint (every later pass is empty too); the batch queued
+    // after it then runs empty.
     if (*reinterpret_cast<volatile uint32_t*>(c->h_conv) == 0) break;
     slot ^= 1;
   }
