@@ -275,6 +275,20 @@ int32_t mptc_set_prior(mptc_ctx* ctx, int32_t net, const float* scores);
  * With the logit columns of W3 and b3 at zero the network then starts as the
  * fixed ranking the base gives and learns corrections to it. */
 int32_t mptc_net_set_base(mptc_ctx* ctx, int32_t net, const float* scores);
+/* Index similarity in the learned search's ranking.  h(L) = the number of
+ * pixels (0..16) whose 2-bit index differs between a block's own index word
+ * and the word of the candidate at offset L (inter: the previous frame's
+ * word; intra: the word the intra search starts from).  With scores set
+ * (host, MPTC_SIM_BINS floats; NULL removes them) the proposals rank offset L
+ * by its network logit (or prior score) + scores[h(L)].  Training is not
+ * affected. */
+#define MPTC_SIM_BINS 17
+int32_t mptc_net_set_similarity(mptc_ctx* ctx, int32_t net, const float* scores);
+/* Statistics for those scores, gathered by mptc_collect_pairs since recording
+ * was last enabled: out[h] = admitted offsets with h differing indices over
+ * the blocks that searched the window, out[MPTC_SIM_BINS + h] = those that
+ * passed the accept test (2 x MPTC_SIM_BINS values). */
+int32_t mptc_get_similarity_counts(mptc_ctx* ctx, int32_t net, uint64_t* out);
 
 /* Allocation-free variant of mptc_encode_frame_device (no counterpart in the
  * reference): for callers that cannot allocate device memory -- code inside

@@ -150,6 +150,8 @@ struct NetState {
   bool use_prior = false;
   float* base = nullptr;   // kNetOut fixed scores added to the logits (seed outputs 0)
   bool use_base = false;
+  float* sim = nullptr;    // kSimBins scores by index difference, added in the search's ranking
+  bool use_sim = false;
   uint16_t* pk = nullptr;  // packed inference image (net_infer.h)
 };
 
@@ -229,6 +231,7 @@ struct mptc_ctx {
   uint32_t* d_seen = nullptr;
   int* d_errs = nullptr;
   uint8_t* d_tflag = nullptr;  // learned intra passes: tiles to redo, by pass parity (2 x max_total)
+  unsigned long long* d_simcnt = nullptr;  // per network: tries, then accepts, by index difference
   // profiling
   bool prof = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -336,14 +339,14 @@ int net_alloc(NetState& n) {
   int rc;
   if ((rc = dalloc(&n.w, kNetParams)) || (rc = dalloc(&n.wb, kNetParams)) || (rc = dalloc(&n.m, kNetParams)) ||
       (rc = dalloc(&n.v, kNetParams)) || (rc = dalloc(&n.g, kNetParams)) || (rc = dalloc(&n.prior, kNetLogits)) ||
-      (rc = dalloc(&n.pk, kPkSize)) || (rc = dalloc(&n.base, kNetOut)))
+      (rc = dalloc(&n.pk, kPkSize)) || (rc = dalloc(&n.base, kNetOut)) || (rc = dalloc(&n.sim, kSimBins)))
     return rc;
   if (hipMemset(n.base, 0, sizeof(float) * kNetOut) != hipSuccess) return fail(MPTC_EHIP, "hipMemset failed");
   return MPTC_OK;
 }
 
 void net_free(NetState& n) {
-  void* b[] = {n.w, n.wb, n.m, n.v, n.g, n.prior, n.pk, n.base};
+  void* b[] = {n.w, n.wb, n.m, n.v, n.g, n.prior, n.pk, n.base, n.sim};
   for (void* p : b)
     if (p) (void)hipFree(p);
   n = NetState();
@@ -633,7 +636,8 @@ int infer_q(mptc_ctx* c, hipStream_t s, const NetState& n, const uint16_t* X, in
   return launch(c, s, KF_NET_L3, [&] {
     auto kern = Y ? k_net_l3<4, 9> : kk <= 4 ? k_net_l3<4, 8> : kk <= 8 ? k_net_l3<8, 8> : k_net_l3<16, 8>;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kInferWaves), 0, s, h2p, rows, n.pk, Y, c->d_fr, c->d_sid, net,
-                       S, kk, need, prop);
+                       S, kk, need, prop, (!Y && n.use_sim) ? n.sim : nullptr, net == 0 ? c->d_rw : c->d_cw,
+                       c->d_own);
   });
 }
 
@@ -646,7 +650,8 @@ int proposals_q(mptc_ctx* c, hipStream_t s, int net, const uint8_t* need, int16_
   if (n.use_prior)
     return launch(c, s, KF_TOP_INTRA, [&] {
       hipLaunchKernelGGL(k_top_scores, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, total, net,
-                         c->cfg.search_area, c->top_k, n.prior, 1L, 0L, need, prop);
+                         c->cfg.search_area, c->top_k, n.prior, 1L, 0L, need, prop, n.use_sim ? n.sim : nullptr,
+                         net == 0 ? c->d_rw : c->d_cw, c->d_own);
     });
   if (!x_done) {
     int rc = launch(c, s, KF_NET_INPUT, [&] {
@@ -1040,7 +1045,7 @@ void mptc_destroy(mptc_ctx* c) {
   c->nw.release();
   c->ew.release();
   void* lbufs[] = {c->d_h2p, c->d_pinter, c->d_prop, c->d_seen, c->d_errs, c->d_amask[0], c->d_amask[1],
-                   c->d_rbt,  c->d_cbt,  c->d_pm,  c->d_tflag};
+                   c->d_rbt,  c->d_cbt,  c->d_pm,  c->d_tflag, c->d_simcnt};
   for (void* p : lbufs)
     if (p) (void)hipFree(p);
   if (c->h_stats) (void)hipHostFree(c->h_stats);
@@ -1452,7 +1457,13 @@ int32_t mptc_record_pairs(mptc_ctx* c, int32_t enable) {
   if (enable && !c->d_amask[0]) {
     int rc;
     const size_t nb = (size_t)c->max_total * 16;
-    if ((rc = dalloc(&c->d_amask[0], nb)) || (rc = dalloc(&c->d_amask[1], nb))) return rc;
+    if ((rc = dalloc(&c->d_amask[0], nb)) || (rc = dalloc(&c->d_amask[1], nb)) ||
+        (rc = dalloc(&c->d_simcnt, 4 * kSimBins)))
+      return rc;
+  }
+  if (enable) {  // a new recording starts the index-similarity statistics afresh
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemset(c->d_simcnt, 0, sizeof(unsigned long long) * 4 * kSimBins));
   }
   if ((enable != 0) != c->record) c->graphs.destroy();
   c->record = enable != 0;
@@ -1470,6 +1481,10 @@ int32_t mptc_collect_pairs(mptc_ctx* c, int32_t net, uint8_t* d_px, uint32_t* d_
   hipLaunchKernelGGL(k_collect_masks, dim3(grid_for(c->total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, c->total,
                      net, c->cfg.search_area, c->d_need, c->d_amask[net], c->d_own, d_px, d_target, d_eps_out,
                      d_count, capacity);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_sim_counts, dim3(grid_for(c->total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid, c->total, net,
+                     c->cfg.search_area, c->d_need, c->d_amask[net], c->d_own, c->d_rw, c->d_inter,
+                     c->d_simcnt + 2 * kSimBins * net);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s));
   return MPTC_OK;
@@ -1529,6 +1544,29 @@ int32_t mptc_net_set_base(mptc_ctx* c, int32_t net, const float* scores) {
   }
   HIP_TRY(hipStreamSynchronize(s));
   c->graphs.destroy();
+  return MPTC_OK;
+}
+
+int32_t mptc_net_set_similarity(mptc_ctx* c, int32_t net, const float* scores) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  NetState& n = c->net[net];
+  if (scores) {
+    HIP_TRY(hipMemcpyAsync(n.sim, scores, sizeof(float) * kSimBins, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  n.use_sim = scores != nullptr;
+  c->graphs.destroy();
+  return MPTC_OK;
+}
+
+int32_t mptc_get_similarity_counts(mptc_ctx* c, int32_t net, uint64_t* out) {
+  int rc = check_net(c, net);
+  if (rc) return rc;
+  if (!out) return fail(MPTC_EINVAL, "null argument");
+  if (!c->d_simcnt) return fail(MPTC_ESTATE, "no recording has been made");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(out, c->d_simcnt + 2 * kSimBins * net, sizeof(uint64_t) * 2 * kSimBins, hipMemcpyDeviceToHost));
   return MPTC_OK;
 }
 

@@ -211,6 +211,23 @@ __device__ __forceinline__ WinMask win_mask(int net, int S, int bx, int by, int 
   return m;
 }
 
+// Index similarity of a candidate: the number of pixels (0..16) whose 2-bit
+// index differs between the block's own index word and the candidate word.
+constexpr int kSimBins = 17;
+__device__ __forceinline__ int index_diff(uint32_t a, uint32_t b) {
+  const uint32_t x = a ^ b;
+  return __popc((x | (x >> 1)) & 0x55555555u);
+}
+
+// The candidate block of offset L (logit index) for block (bx, by) of frame d:
+// inter (net 0) rows dy = L / 16 - 8, intra (net 1) rows by - L / 16; columns
+// dx = L % 16 - 8.  L must be admitted by win_mask.
+__device__ __forceinline__ int offset_block(const FrameDev& d, int net, int bx, int by, int L) {
+  const int dx = (L & 15) - 8, r = L >> 4;
+  const int y = net == 0 ? by + r - 8 : by - r;
+  return d.base + y * d.bw + bx + dx;
+}
+
 // Rank key of an admitted offset's score: the fp32 score (a NaN counts as
 // -FLT_MAX) mapped to an order-preserving unsigned integer, its low 8 bits
 // replaced by 255 - L.  A larger key ranks first; scores equal to 24 bits rank
@@ -274,6 +291,9 @@ __device__ __forceinline__ void key_insert(uint32_t (&t)[K], uint32_t key) {
 }
 
 // Layer 3.  NT = 9: writes the 262 outputs of each row to Y (row-major).
+// With sim (kSimBins scores) the top-k adds, to each admitted offset's logit,
+// sim[index_diff(own word, candidate word)] (words: the candidates' current
+// index words, own: the blocks' fits).
 // NT = 8 (the 256 logit columns only; the endpoint-seed outputs are not
 // computed): for each block g of the batch (rows = total) that `need`
 // admits (need null: every block of an inter frame), the top kk <= K
@@ -284,13 +304,17 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
                                                              const FrameDev* __restrict__ fr,
                                                              const uint8_t* __restrict__ sid, int net, int S, int kk,
                                                              const uint8_t* __restrict__ need,
-                                                             int16_t* __restrict__ prop) {
+                                                             int16_t* __restrict__ prop, const float* __restrict__ sim,
+                                                             const uint32_t* __restrict__ words,
+                                                             const uint64_t* __restrict__ own) {
   static_assert(NT == 8 || NT == 9, "8 logit tiles, or all 9 output tiles");
   __shared__ uint4 sW3[NT * 32 * kNetHid / 8];  // 128 or 144 KiB, rows of 32 chunks, swizzled
   __shared__ __attribute__((aligned(16))) float sb[NT * 32];
+  __shared__ float s_sim[kSimBins];
   stage_rows(sW3, reinterpret_cast<const uint4*>(pk + kPkW3), NT * 32, kNetHid / 8, true);
   const float* bias = reinterpret_cast<const float*>(pk + kPkBias) + 2 * kNetHid;
   for (int i = threadIdx.x; i < NT * 32; i += blockDim.x) sb[i] = bias[i];
+  if (threadIdx.x < kSimBins) s_sim[threadIdx.x] = sim ? sim[threadIdx.x] : 0.0f;
   __syncthreads();
   const int lane = threadIdx.x & 63, m = lane & 31, h = lane >> 5;
   const int n_tiles = (rows + 31) / 32;
@@ -328,11 +352,28 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
     // K; lane h = 0 merges the partner lane's (l + 32) list and writes
     bool act = row < rows;
     uint32_t vw[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+    int fbx = 0, fby = 0;
+    const FrameDev* dp = fr;
     if (act) {
-      const FrameDev& d = fr[sid[row]];
-      const int k = row - d.base;
-      act = need ? need[row] != 0 : !d.is_intra;
-      if (act) admitted_words(win_mask(net, S, k % d.bw, k / d.bw, d.bw, d.bh), vw);
+      dp = &fr[sid[row]];
+      const int k = row - dp->base;
+      fbx = k % dp->bw;
+      fby = k / dp->bw;
+      act = need ? need[row] != 0 : !dp->is_intra;
+      if (act) admitted_words(win_mask(net, S, fbx, fby, dp->bw, dp->bh), vw);
+    }
+    if (sim && act) {
+      // index similarity: each admitted offset's score gains sim[number of
+      // pixels whose index differs between the own word and the candidate's]
+      const uint32_t ow = (uint32_t)(own[row] >> 32);
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int bit = 8 * (i >> 2) + 4 * h + (i & 3);
+          if ((vw[nb] >> bit) & 1u)
+            acc[nb][i] += s_sim[index_diff(ow, words[offset_block(*dp, net, fbx, fby, nb * 32 + bit)])];
+        }
     }
     uint32_t top[K];
 #pragma unroll
@@ -358,23 +399,34 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
 }
 
 // Top-k admitted offsets from fixed scores (one shared prior, ys_g = 0) or
-// transposed logits (ys_l = rows, ys_g = 1), ranked as k_net_l3's fused top-k.
+// transposed logits (ys_l = rows, ys_g = 1), ranked as k_net_l3's fused top-k
+// (with the same index-similarity term when sim is given).
 __global__ void __launch_bounds__(256) k_top_scores(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
                                                     int total, int net, int S, int kk, const float* __restrict__ Ys,
                                                     long ys_l, long ys_g, const uint8_t* __restrict__ need,
-                                                    int16_t* __restrict__ prop) {
+                                                    int16_t* __restrict__ prop, const float* __restrict__ sim,
+                                                    const uint32_t* __restrict__ words,
+                                                    const uint64_t* __restrict__ own) {
+  __shared__ float s_sim[kSimBins];
+  if (threadIdx.x < kSimBins) s_sim[threadIdx.x] = sim ? sim[threadIdx.x] : 0.0f;
+  __syncthreads();
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const FrameDev& d = fr[sid[g]];
   if (need ? !need[g] : d.is_intra) return;
-  const int k = g - d.base;
-  const WinMask ok = win_mask(net, S, k % d.bw, k / d.bw, d.bw, d.bh);
+  const int k = g - d.base, bx = k % d.bw, by = k / d.bw;
+  const WinMask ok = win_mask(net, S, bx, by, d.bw, d.bh);
   uint32_t t[kMaxTopK];
 #pragma unroll
   for (int i = 0; i < kMaxTopK; ++i) t[i] = 0u;
   const float* y = Ys + (long)g * ys_g;
+  const uint32_t ow = sim ? (uint32_t)(own[g] >> 32) : 0u;
   for (int L = 0; L < kNetLogits; ++L)
-    if (ok(L)) key_insert<kMaxTopK>(t, rank_key(y[(long)L * ys_l], L));
+    if (ok(L)) {
+      float v = y[(long)L * ys_l];
+      if (sim) v += s_sim[index_diff(ow, words[offset_block(d, net, bx, by, L)])];
+      key_insert<kMaxTopK>(t, rank_key(v, L));
+    }
 #pragma unroll
   for (int i = 0; i < kMaxTopK; ++i)
     if (i < kk) prop[(size_t)g * kk + i] = (int16_t)key_offset(t[i]);
@@ -439,6 +491,45 @@ __global__ void __launch_bounds__(256) k_eval_inter(const FrameDev* __restrict__
     }
   }
   wg_count(&s_acc, n_cand, st->inter_cands);
+}
+
+// Index-similarity statistics of a recorded step (the exhaustive search of
+// network `net`'s window): for every block that searched that window and
+// every admitted offset, h = index_diff(own word, candidate word) is counted
+// in cnt[h] and, when the offset passes the accept test (amask), also in
+// cnt[kSimBins + h].  The candidates are the words the learned search sees
+// when it proposes: inter, the previous frame's words (rw); intra, the words
+// the intra search starts from (the accepted inter word where a block reuses
+// one, else the block's own word).
+__global__ void __launch_bounds__(256) k_sim_counts(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid,
+                                                    int total, int net, int S, const uint8_t* __restrict__ need,
+                                                    const uint16_t* __restrict__ amask,
+                                                    const uint64_t* __restrict__ own, const uint32_t* __restrict__ rw,
+                                                    const uint64_t* __restrict__ inter_dec,
+                                                    unsigned long long* __restrict__ cnt) {
+  __shared__ unsigned int s_c[2 * kSimBins];
+  for (int i = threadIdx.x; i < 2 * kSimBins; i += blockDim.x) s_c[i] = 0u;
+  __syncthreads();
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < total) {
+    const FrameDev& d = fr[sid[g]];
+    if (net == 0 ? !d.is_intra : need[g] != 0) {
+      const int k = g - d.base, bx = k % d.bw, by = k / d.bw;
+      const WinMask ok = win_mask(net, S, bx, by, d.bw, d.bh);
+      const uint32_t ow = (uint32_t)(own[g] >> 32);
+      for (int L = 0; L < kNetLogits; ++L) {
+        if (!ok(L)) continue;
+        const int q = offset_block(d, net, bx, by, L);
+        const uint32_t c = net == 0 ? rw[q] : need[q] ? (uint32_t)(own[q] >> 32) : dec_word(inter_dec[q]);
+        const int hd = index_diff(ow, c);
+        atomicAdd(&s_c[hd], 1u);
+        if ((amask[(size_t)g * 16 + (L >> 4)] >> (L & 15)) & 1u) atomicAdd(&s_c[kSimBins + hd], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2 * kSimBins; i += blockDim.x)
+    if (s_c[i]) atomicAdd(&cnt[i], (unsigned long long)s_c[i]);
 }
 
 }  // namespace mptc

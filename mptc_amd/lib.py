@@ -66,10 +66,19 @@ class TrainCfg(C.Structure):
 
 NET_INTER, NET_INTRA = 0, 1
 NET_IN, NET_HIDDEN, NET_OUT, NET_LOGITS, NET_PARAMS = 48, 256, 262, 256, 145670
+SIM_BINS = 17  # differing pixel indices 0..16 (mptc_net_set_similarity)
 # offsets in the flat parameter vector: W1 [48][256], b1, W2 [256][256], b2, W3 [256][262], b3
 NET_OFF_W3 = NET_IN * NET_HIDDEN + NET_HIDDEN + NET_HIDDEN * NET_HIDDEN + NET_HIDDEN
 NET_OFF_B3 = NET_OFF_W3 + NET_HIDDEN * NET_OUT
 assert NET_OFF_B3 + NET_OUT == NET_PARAMS
+
+
+def similarity_scores(tries: np.ndarray, accepts: np.ndarray) -> np.ndarray:
+    """Log of the smoothed acceptance rate by number of differing indices:
+    an additive ranking score (log P(accept | h)), from similarity_counts."""
+    t = np.asarray(tries, np.float64)
+    a = np.asarray(accepts, np.float64)
+    return np.log((a + 0.5) / (t + 1.0)).astype(np.float32)
 
 
 def zero_logit_layer(params: np.ndarray) -> np.ndarray:
@@ -135,6 +144,8 @@ def load() -> C.CDLL:
     L.mptc_set_prior.argtypes = [_vp, C.c_int32, _vp]
     L.mptc_get_proposals.argtypes = [_vp, C.c_int32, _vp, C.c_int64]
     L.mptc_net_set_base.argtypes = [_vp, C.c_int32, _vp]
+    L.mptc_net_set_similarity.argtypes = [_vp, C.c_int32, _vp]
+    L.mptc_get_similarity_counts.argtypes = [_vp, C.c_int32, _vp]
     L.mptc_frame_sse.argtypes = [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp]
     L.mptc_encode_frame_noalloc.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                             C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
@@ -342,6 +353,23 @@ class Encoder:
             a = np.ascontiguousarray(scores, np.float32)
             assert a.shape == (256,)
             _check(self.L.mptc_net_set_base(self.h, net, a.ctypes.data))
+
+    def net_set_similarity(self, net: int, scores: np.ndarray | None):
+        """Scores by the number of differing pixel indices (17) added in the
+        learned search's ranking (None removes them)."""
+        if scores is None:
+            _check(self.L.mptc_net_set_similarity(self.h, net, None))
+        else:
+            a = np.ascontiguousarray(scores, np.float32)
+            assert a.shape == (SIM_BINS,)
+            _check(self.L.mptc_net_set_similarity(self.h, net, a.ctypes.data))
+
+    def similarity_counts(self, net: int) -> tuple[np.ndarray, np.ndarray]:
+        """(tries, accepts) by number of differing indices, gathered by
+        collect_pairs since recording was enabled."""
+        out = np.zeros(2 * SIM_BINS, np.uint64)
+        _check(self.L.mptc_get_similarity_counts(self.h, net, out.ctypes.data))
+        return out[:SIM_BINS].copy(), out[SIM_BINS:].copy()
 
     def set_prior(self, net: int, scores: np.ndarray | None):
         if scores is None:

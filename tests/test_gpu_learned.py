@@ -498,3 +498,134 @@ def test_learned_intra_tile_shapes_agree(env, orc, name, thr):
             e.set_tuning(itile_w=12)
         finally:
             e.close()
+
+
+def _index_diff(a, b):
+    """Pixels whose 2-bit index differs between index words a and b."""
+    x = (np.asarray(a, np.uint32) ^ np.asarray(b, np.uint32)).astype(np.uint32)
+    v = (x | (x >> np.uint32(1))) & np.uint32(0x55555555)
+    return np.array([bin(int(t)).count("1") for t in np.ravel(v)]).reshape(np.shape(v))
+
+
+def _cand_block(net, bx, by, bw, L):
+    dx, r = L % 16 - 8, L // 16
+    y = by + r - 8 if net == 0 else by - r
+    return y * bw + bx + dx
+
+
+@pytest.mark.parametrize("source", ["network", "prior"])
+def test_similarity_scores_enter_the_ranking(env, source):
+    """With index-similarity scores set, the proposals are the top-k admitted
+    offsets of (logit or prior score) + sim[h], h = the pixels whose index
+    differs between the block's own word and the candidate's word (inter: the
+    previous frame's word; intra: the word the intra search starts from)."""
+    hip, lib = env
+    spec = dict(SEQUENCES["c1_s8_t8"])
+    frames = lib.synth_frames(synth_cfg(spec), 0, 2)
+    seq = Seq(hip, frames)
+    S, k = 8, 8
+    enc = lib.Encoder(seq.w, seq.h, S, 0)
+    rng = np.random.default_rng(77)
+    sims = [rng.normal(0, 4, 17).astype(np.float32) for _ in (0, 1)]
+    for net in (0, 1):
+        if source == "network":
+            enc.net_init(net, 51 + net)
+        else:
+            enc.set_prior(net, nearest_prior(net))
+        enc.net_set_similarity(net, sims[net])
+    enc.set_search_mode(2, k)
+    kinds = hip.Buffer(seq.nb)
+    for f in range(2):
+        enc.encode_frame_device(seq.rgb_at(f), seq.w, seq.h, f == 0, None if f == 0 else seq.phys[0], seq.phys[f],
+                                seq.motion[f], seq.unique, seq.n_unique, seq.eps[f], kinds)
+    kind = kinds.download(np.uint8, seq.nb)
+    bw, bh = seq.w // 4, seq.h // 4
+    own = (enc.fit(frames[1]) >> np.uint64(32)).astype(np.uint32)
+    prev = (seq.phys[0].download(np.uint64, seq.nb) >> np.uint64(32)).astype(np.uint32)
+    cur = (seq.phys[1].download(np.uint64, seq.nb) >> np.uint64(32)).astype(np.uint32)
+    start = np.where(kind == 2, cur, own)  # inter-reused blocks start the intra search at their reused word
+    px = frames[1].reshape(bh, 4, bw, 4, 3).transpose(0, 2, 1, 3, 4).reshape(seq.nb, 48).copy()
+    d_px, d_y = hip.Buffer.from_array(px), hip.Buffer(seq.nb * 262 * 4)
+    for net in (0, 1):
+        props = enc.proposals(net, seq.nb, k)
+        if source == "network":
+            enc.net_forward(net, d_px, seq.nb, d_y)
+            y = d_y.download(np.float32, seq.nb * 262).reshape(seq.nb, 262)[:, :256]
+        else:
+            y = np.tile(nearest_prior(net), (seq.nb, 1))
+        words = prev if net == 0 else start
+        checked = 0
+        for g in range(0, seq.nb, 3):
+            if net == 1 and kind[g] == 2:
+                continue
+            bx, by = g % bw, g // bw
+            cand = np.flatnonzero(_admitted(net, S, bx, by, bw, bh))
+            q = np.array([_cand_block(net, bx, by, bw, L) for L in cand], dtype=np.int64)
+            h = _index_diff(np.full(len(q), own[g], np.uint32), words[q]).astype(np.int64)
+            score = (y[g, cand] + sims[net][h]).astype(np.float32)
+            order = cand[np.argsort(-rank_keys(score, cand), kind="stable")][:k]
+            exp = np.full(k, -1)
+            exp[:len(order)] = order
+            assert np.array_equal(props[g], exp), (source, net, g, props[g], exp)
+            checked += 1
+        assert checked > 0
+    enc.close()
+
+
+def test_similarity_counts_match_the_oracle(env, orc):
+    """The statistics gathered while recording: per number of differing
+    indices h, the admitted offsets of every searching block (tries) and
+    those passing the accept test, against the oracle's exact evaluation of
+    the same candidates (inter: previous frame's words; intra: acceptance
+    against the final words, h against the starting words)."""
+    from oracle import SynthCfg
+    hip, lib = env
+    S, thr = 6, 6
+    frames = orc.synth_frames(SynthCfg(128, 64, 31, 5, 1, 3, 8, 0), 2)
+    seq = Seq(hip, frames)
+    enc = lib.Encoder(seq.w, seq.h, S, thr)
+    enc.record_pairs(True)
+    kinds = hip.Buffer(seq.nb)
+    cap = seq.nb
+    bufs = [dict(px=hip.Buffer(48 * cap), tgt=hip.Buffer(32 * cap), eps=hip.Buffer(6 * cap),
+                 cnt=hip.Buffer.from_array(np.zeros(1, np.uint32))) for _ in (0, 1)]
+    kind_f = []
+    for f in range(2):
+        enc.encode_frame_device(seq.rgb_at(f), seq.w, seq.h, f == 0, None if f == 0 else seq.phys[0], seq.phys[f],
+                                seq.motion[f], seq.unique, seq.n_unique, seq.eps[f], kinds)
+        kind_f.append(kinds.download(np.uint8, seq.nb))
+        for net in (0, 1):
+            b = bufs[net]
+            enc.collect_pairs(net, b["px"], b["tgt"], b["eps"], b["cnt"], cap)
+    bw, bh = seq.w // 4, seq.h // 4
+    want = {net: (np.zeros(17, np.int64), np.zeros(17, np.int64)) for net in (0, 1)}
+    for f in range(2):
+        own = orc.fit_frame(frames[f])
+        ow = (own >> np.uint64(32)).astype(np.uint32)
+        fin = (seq.phys[f].download(np.uint64, seq.nb) >> np.uint64(32)).astype(np.uint32)
+        start = np.where(kind_f[f] == 2, fin, ow)
+        prev = None if f == 0 else (seq.phys[0].download(np.uint64, seq.nb) >> np.uint64(32)).astype(np.uint32)
+        for g in range(seq.nb):
+            bx, by = g % bw, g // bw
+            pxg = frames[f][4 * by:4 * by + 4, 4 * bx:4 * bx + 4].reshape(48)
+            for net in (0, 1):
+                if net == 0 and f == 0:
+                    continue
+                if net == 1 and kind_f[f][g] == 2:
+                    continue
+                for L in np.flatnonzero(_admitted(net, S, bx, by, bw, bh)):
+                    q = _cand_block(net, bx, by, bw, L)
+                    hw = prev[q] if net == 0 else start[q]
+                    h = int(_index_diff(ow[g], hw))
+                    cw = prev[q] if net == 0 else fin[q]
+                    ok, e = orc.eval_candidate(int(own[g]), pxg, int(cw))[:2]
+                    want[net][0][h] += 1
+                    if ok and e <= thr:
+                        want[net][1][h] += 1
+    for net in (0, 1):
+        tries, acc = enc.similarity_counts(net)
+        log(f"net {net}: tries {tries.tolist()} accepts {acc.tolist()}")
+        assert np.array_equal(tries.astype(np.int64), want[net][0]), net
+        assert np.array_equal(acc.astype(np.int64), want[net][1]), net
+        assert acc.sum() > 0
+    enc.close()
