@@ -13,9 +13,11 @@ excluded); the K timed steps walk the sequence from frame 0.  `value` is the
 plain path, bit-exact to the reference encoder.
 
 The learned path is trained inside the run: the plain search records which
-offsets pass its accept test on 1080p training sequences of other seeds, both
-proposal networks are trained on the GPU, and the same K steps are encoded
-with top-8 proposals (exact acceptance).  Prints one JSON line.
+offsets pass its accept test on 1080p training sequences of other seeds (and
+how often an offset passes by the number of pixel indices its word differs
+from the block's own), both proposal networks are trained on the GPU, and the
+same K steps are encoded with top-4 proposals (the inter ranking adding the
+index-similarity scores; exact acceptance).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -68,7 +70,7 @@ def parse():
     ap.add_argument("--no-configs", action="store_true", help="skip the config 3 (4K) and config 4 (12 streams) runs")
     ap.add_argument("--config-steps", type=int, default=4, help="timed frames per config 3 / config 4 run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
-    ap.add_argument("--top-k", type=int, default=8)
+    ap.add_argument("--top-k", type=int, default=4)
     ap.add_argument("--no-threshold8", action="store_true", help="skip the learned frontier at threshold 8")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--workload", default=WORKLOAD)
@@ -604,7 +606,7 @@ def learned_point(lib, enc, R, a, k, plain, with_quality=True):
     reused = w["n_inter"] + w["n_intra"]
     pt = {"top_k": k, "value": round(v, 4), "ms_per_step": round(t / a.steps * 1e3, 4),
           "speedup_vs_plain": round(v / plain["value"], 3), "reused_blocks": reused,
-          "n_inter": w["n_inter"], "n_intra": w["n_intra"],
+          "n_inter": w["n_inter"], "n_intra": w["n_intra"], "n_unique": w["n_unique"],
           "reuse_recall": round(reused / max(1, reused_plain), 4),
           "unique_blocks_delta_pct": round(100.0 * (w["n_unique"] - plain["work"]["n_unique"])
                                            / max(1, plain["work"]["n_unique"]), 3)}
@@ -615,23 +617,47 @@ def learned_point(lib, enc, R, a, k, plain, with_quality=True):
     return pt, w
 
 
-def learned_frontier(hip, lib, enc, R, a, plain, prior_alpha):
-    """k in {4, 8, 16} for the trained networks (already set), the fixed
-    nearest-first ranking and untrained networks, in that order."""
+def learned_frontier(hip, lib, enc, R, a, plain, prior_alpha, sims):
+    """k in {4, 8, 16} for: the trained networks with the inter network's
+    index-similarity scores (the headline configuration, already set);
+    untrained networks with the same scores (the base ranking prior_alpha x
+    nearest-first + similarity: the method without a trained network); the
+    trained networks with similarity scores on both networks and on neither;
+    and the plain nearest-first ranking."""
     fr = {}
-    for method in ("trained", "nearest_prior", "untrained"):
-        if method == "nearest_prior":
+    for method in ("trained+inter_similarity", "untrained+inter_similarity", "trained+similarity", "trained",
+                   "nearest_prior"):
+        if method == "untrained+inter_similarity":
+            saved = [enc.net_params(net).copy() for net in (0, 1)]
+            for net in (0, 1):
+                enc.net_init(net, 99 + net)
+                enc.net_set_params(net, lib.zero_logit_layer(enc.net_params(net)))
+                enc.net_set_base(net, prior_alpha * nearest_scores(net))
+        elif method == "trained+similarity":
+            for net in (0, 1):
+                enc.net_set_params(net, saved[net])
+                enc.net_set_similarity(net, sims[net])
+        elif method == "trained":
+            for net in (0, 1):
+                enc.net_set_similarity(net, None)
+        elif method == "nearest_prior":
             for net in (0, 1):
                 enc.set_prior(net, nearest_scores(net))
-        elif method == "untrained":
-            for net in (0, 1):
-                enc.set_prior(net, None)
-                enc.net_init(net, 99 + net)
-                if prior_alpha > 0:  # the same start as training: the base ranking, untrained corrections
-                    enc.net_set_params(net, lib.zero_logit_layer(enc.net_params(net)))
-                    enc.net_set_base(net, prior_alpha * nearest_scores(net))
         fr[method] = {str(k): learned_point(lib, enc, R, a, k, plain)[0] for k in (4, 8, 16)}
+    for net in (0, 1):
+        enc.set_prior(net, None)
     return fr
+
+
+def similarity_of(lib, enc):
+    """Index-similarity scores from the statistics of the last recording."""
+    out, sims = {}, {}
+    for net in (0, 1):
+        tries, acc = enc.similarity_counts(net)
+        sims[net] = lib.similarity_scores(tries, acc)
+        out["inter" if net == 0 else "intra"] = {"tries": tries.tolist(), "accepts": acc.tolist(),
+                                                 "scores": [round(float(v), 3) for v in sims[net]]}
+    return sims, out
 
 
 PRIOR_ALPHA = 0.25  # base scores = 0.25 x nearest-first ranking (corrections learned on top)
@@ -647,9 +673,12 @@ def measure_learned(hip, lib, enc, R, spec, a, value, work, psnr_plain, tuning):
     plain = {"value": value, "work": work, "psnr_db": psnr_plain}
     note("learned: collecting pairs and training")
     bufs, info = collect_pairs(hip, lib, enc, spec, TRAIN_SEEDS)
+    sims, info["similarity"] = similarity_of(lib, enc)
     info.update(train_networks(lib, enc, bufs, a.train_steps, PRIOR_ALPHA))
     free_pairs(bufs)
-    learned = {"top_k": a.top_k, "training": info}
+    enc.net_set_similarity(0, sims[0])  # the inter network's ranking gains the similarity scores
+    learned = {"top_k": a.top_k, "training": info,
+               "proposals": "trained networks; inter ranking + index-similarity scores"}
     enc.set_search_mode(lib.SEARCH_LEARNED, a.top_k)
     if not a.no_autotune:
         ltune, _ = R.autotune()
@@ -670,7 +699,15 @@ def measure_learned(hip, lib, enc, R, spec, a, value, work, psnr_plain, tuning):
                                       "unit": "TFLOP/s", "frac": round(ach / MFMA_BF16_PEAK, 4),
                                       "work_per_launch": f"{nb} blocks x {NET_FLOP_L3} flop (logits) + top-k"}
     note("learned: k frontier")
-    learned["frontier"] = learned_frontier(hip, lib, enc, R, a, plain, PRIOR_ALPHA)
+    learned["frontier"] = learned_frontier(hip, lib, enc, R, a, plain, PRIOR_ALPHA, sims)
+    # against the method without the network and the plain nearest-first ranking, at the same k
+    for ref in ("untrained+inter_similarity", "nearest_prior"):
+        p = learned["frontier"][ref][str(a.top_k)]
+        learned[f"vs_{ref}"] = {"speedup": round(learned["value"] / p["value"], 3),
+                                "reused_blocks_delta_pct": round(100.0 * (learned["reused_blocks"] - p["reused_blocks"])
+                                                                 / max(1, p["reused_blocks"]), 3),
+                                "unique_blocks_delta_pct": round(100.0 * (lw["n_unique"] - p["n_unique"])
+                                                                 / max(1, p["n_unique"]), 3)}
     if a.no_threshold8:
         return learned
     note("learned: threshold 8")
@@ -683,15 +720,17 @@ def measure_learned(hip, lib, enc, R, spec, a, value, work, psnr_plain, tuning):
     w8 = work_of(R.run(a.steps, collect=True))
     plain8 = {"value": a.steps * nb / t8 / 1e6, "work": w8, "psnr_db": R.quality(a.steps)}
     bufs, info8 = collect_pairs(hip, lib, enc, spec8, TRAIN_SEEDS[:4])
+    sims8, info8["similarity"] = similarity_of(lib, enc)
     info8.update(train_networks(lib, enc, bufs, a.train_steps, PRIOR_ALPHA))
     free_pairs(bufs)
+    enc.net_set_similarity(0, sims8[0])
     if "tuning" in learned:
         enc.set_tuning(**learned["tuning"])
     learned["threshold8"] = {
         "plain": {"value": round(plain8["value"], 4), "reused_blocks": w8["n_inter"] + w8["n_intra"],
                   "n_unique": w8["n_unique"], "psnr_db": round(plain8["psnr_db"], 4)},
         "training": info8,
-        "frontier": learned_frontier(hip, lib, enc, R, a, plain8, PRIOR_ALPHA)}
+        "frontier": learned_frontier(hip, lib, enc, R, a, plain8, PRIOR_ALPHA, sims8)}
     enc.set_params(spec["S"], spec["thr"])
     return learned
 
