@@ -311,6 +311,10 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
   __shared__ uint4 sW3[NT * 32 * kNetHid / 8];  // 128 or 144 KiB, rows of 32 chunks, swizzled
   __shared__ __attribute__((aligned(16))) float sb[NT * 32];
   __shared__ float s_sim[kSimBins];
+  // per wave: the candidate words of a tile of 32 blocks of one frame row,
+  // 16 window rows x (32 + 15) columns (similarity term, search mode only)
+  constexpr int kWinCols = 32 + kNetWin - 1;
+  __shared__ uint32_t s_win[NT == 8 ? kInferWaves : 1][NT == 8 ? kNetWin * kWinCols : 1];
   stage_rows(sW3, reinterpret_cast<const uint4*>(pk + kPkW3), NT * 32, kNetHid / 8, true);
   const float* bias = reinterpret_cast<const float*>(pk + kPkBias) + 2 * kNetHid;
   for (int i = threadIdx.x; i < NT * 32; i += blockDim.x) sb[i] = bias[i];
@@ -362,18 +366,51 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
       act = need ? need[row] != 0 : !dp->is_intra;
       if (act) admitted_words(win_mask(net, S, fbx, fby, dp->bw, dp->bh), vw);
     }
-    if (sim && act) {
-      // index similarity: each admitted offset's score gains sim[number of
-      // pixels whose index differs between the own word and the candidate's]
-      const uint32_t ow = (uint32_t)(own[row] >> 32);
-#pragma unroll
-      for (int nb = 0; nb < 8; ++nb)
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int bit = 8 * (i >> 2) + 4 * h + (i & 3);
-          if ((vw[nb] >> bit) & 1u)
-            acc[nb][i] += s_sim[index_diff(ow, words[offset_block(*dp, net, fbx, fby, nb * 32 + bit)])];
+    if (NT == 8 && sim) {
+      // Index similarity: each admitted offset's score gains sim[number of
+      // pixels whose index differs between the own word and the candidate's].
+      // When the tile's 32 blocks lie in one row of one frame, their windows'
+      // words are staged in LDS once (coalesced loads); otherwise each lane
+      // reads its candidates' words from memory.
+      const int g0 = tile * 32;
+      bool staged = false;
+      int bx_lo = 0;
+      if (g0 + 31 < rows) {
+        const int f0 = sid[g0];
+        const FrameDev& d0 = fr[f0];
+        const int k0 = g0 - d0.base;
+        if (sid[g0 + 31] == f0 && k0 / d0.bw == (k0 + 31) / d0.bw) {
+          staged = true;
+          bx_lo = k0 % d0.bw;
+          const int by0 = k0 / d0.bw;
+          uint32_t* wv = s_win[threadIdx.x >> 6];
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the last tile's reads are done
+          __builtin_amdgcn_wave_barrier();
+          for (int idx = lane; idx < kNetWin * kWinCols; idx += 64) {
+            const int r = idx / kWinCols, col = idx - r * kWinCols;
+            const int y = net == 0 ? by0 + r - 8 : by0 - r, x = bx_lo - 8 + col;
+            wv[idx] = (y >= 0 && y < d0.bh && x >= 0 && x < d0.bw) ? words[d0.base + y * d0.bw + x] : 0u;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
+      }
+      if (act) {
+        const uint32_t ow = (uint32_t)(own[row] >> 32);
+        const uint32_t* wv = s_win[threadIdx.x >> 6] + (fbx - bx_lo);
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb)
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const int bit = 8 * (i >> 2) + 4 * h + (i & 3), L = nb * 32 + bit;
+            if ((vw[nb] >> bit) & 1u) {
+              const uint32_t w = staged ? wv[(L >> 4) * kWinCols + (L & 15)]
+                                        : words[offset_block(*dp, net, fbx, fby, L)];
+              acc[nb][i] += s_sim[index_diff(ow, w)];
+            }
+          }
+      }
     }
     uint32_t top[K];
 #pragma unroll
