@@ -11,10 +11,11 @@
 //   intra  L = dy * 16 + (dx + 8),        dy = by - j in [0, 15], dx = i - bx in [-8, 7]
 //          (row dy = 0 holds only dx < 0: the window is causal)
 //
-// Training kernels: k_gemm_bf16, k_net_input, k_net_loss, k_colsum_part/final,
-// k_adam, k_to_bf16, k_collect_masks.  Search: k_frame_net_input (the
-// network input of a batch) and k_lintra_pass (learned intra fixpoint); the
-// inference kernels and the top-k proposal kernels are in net_infer.h.
+// Training kernels: k_train_rows, k_train_wgrad, k_grad_reduce (with the
+// Adam step), k_train_copies, k_collect_masks; k_net_input (the inputs of a
+// forward pass).  Search: k_frame_net_input (the network input of a batch)
+// and k_lintra_pass (learned intra fixpoint); the inference kernels and the
+// top-k proposal kernels are in net_infer.h.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -60,128 +61,6 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
   return u;
 }
 
-// ---------------------------------------------------------------------------
-// GEMM.  Element (m, k) of A is A[m * sam + k * sak]; (k, n) of B is
-// B[k * sbk + n * sbn]; elements outside M x K / K x N read as 0.  A workgroup
-// of 256 threads computes a 64 x 64 tile of C; wave w owns the 32 x 32
-// quadrant (w / 2, w % 2) as 2 x 2 MFMA 16x16x32 tiles.  MFMA operand layout
-// (lane l of the wave): A row l % 16, k (l / 16) * 8 .. +7; B column l % 16,
-// k (l / 16) * 8 .. +7; D rows (l / 16) * 4 .. +3, column l % 16.
-// Split K (gridDim.z > 1, for the weight gradients, whose K is the batch):
-// workgroup z takes a 32-aligned range of k and stores its partial product,
-// without epilogue, to Cf + z * split_stride; k_split_sum adds the partials
-// in a fixed order.
-// ---------------------------------------------------------------------------
-struct GemmEpi {
-  const float* bias;     // [N] or null
-  const float* bias2;    // [N] or null: a second, constant bias added after `bias`
-  int relu;              // max(0, .) before storing
-  const uint16_t* mask;  // bf16 [M x N] (row stride ldm): result zeroed where mask <= 0; or null
-  long ldm;
-  float* Cf;             // fp32 output or null
-  uint16_t* Cb;          // bf16 output or null
-  long ldc, ldcb;        // row strides of Cf, Cb
-  int trans_f;           // Cf stored transposed: Cf[n * ldc + m]
-  long split_stride;     // split K: floats between the partial products
-};
-
-__global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, long sam,
-                                                   long sak, const uint16_t* __restrict__ B, long sbk, long sbn,
-                                                   GemmEpi ep) {
-  // k-contiguous rows (80 bytes: 16-byte aligned fragments, skewed banks)
-  __shared__ __attribute__((aligned(16))) uint16_t sA[64][32 + 8];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[64][32 + 8];  // B transposed: sB[n][k]
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  f32x4_t acc[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  const int kspan = ((K + (int)gridDim.z - 1) / (int)gridDim.z + 31) & ~31;
-  const int kb = (int)blockIdx.z * kspan, ke = min(K, kb + kspan);
-  // 16-byte loads of 8 elements along an operand's contiguous dimension where
-  // the other stride keeps them aligned (k0, m0, n0 are multiples of 8)
-  const bool a16 = ((uintptr_t)A & 15u) == 0, b16 = ((uintptr_t)B & 15u) == 0;
-  const bool a_k = a16 && sak == 1 && (sam & 7) == 0, a_m = a16 && sam == 1 && (sak & 7) == 0;
-  const bool b_k = b16 && sbk == 1 && (sbn & 7) == 0, b_n = b16 && sbn == 1 && (sbk & 7) == 0;
-  for (int k0 = kb; k0 < ke; k0 += 32) {
-    const bool kfull = k0 + 32 <= ke;
-    if (kfull && m0 + 64 <= M && a_k) {  // a row's 8 consecutive k
-      const int mm = tid >> 2, kq = (tid & 3) * 8;
-      *reinterpret_cast<uint4*>(&sA[mm][kq]) =
-          *reinterpret_cast<const uint4*>(A + (long)(m0 + mm) * sam + (k0 + kq));
-    } else if (kfull && m0 + 64 <= M && a_m) {  // 8 consecutive m at one k
-      const int kk = tid >> 3, mq = (tid & 7) * 8;
-      const uint4 v = *reinterpret_cast<const uint4*>(A + (long)(m0 + mq) + (long)(k0 + kk) * sak);
-      uint16_t h[8];
-      __builtin_memcpy(h, &v, 16);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) sA[mq + j][kk] = h[j];
-    } else {
-      // consecutive threads walk the contiguous dimension of each operand
-      for (int e = tid; e < 64 * 32; e += 256) {
-        const int mm = sak == 1 ? e / 32 : e % 64, kk = sak == 1 ? e % 32 : e / 64;
-        const int m = m0 + mm, k = k0 + kk;
-        sA[mm][kk] = (m < M && k < ke) ? A[(long)m * sam + (long)k * sak] : (uint16_t)0;
-      }
-    }
-    if (kfull && n0 + 64 <= N && b_k) {  // a column's 8 consecutive k
-      const int nn = tid >> 2, kq = (tid & 3) * 8;
-      *reinterpret_cast<uint4*>(&sB[nn][kq]) =
-          *reinterpret_cast<const uint4*>(B + (long)(n0 + nn) * sbn + (k0 + kq));
-    } else if (kfull && n0 + 64 <= N && b_n) {  // 8 consecutive n at one k
-      const int kk = tid >> 3, nq = (tid & 7) * 8;
-      const uint4 v = *reinterpret_cast<const uint4*>(B + (long)(k0 + kk) * sbk + (n0 + nq));
-      uint16_t h[8];
-      __builtin_memcpy(h, &v, 16);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) sB[nq + j][kk] = h[j];
-    } else {
-      for (int e = tid; e < 64 * 32; e += 256) {
-        const int nn = sbn == 1 ? e % 64 : e / 32, kk = sbn == 1 ? e / 64 : e % 32;
-        const int k = k0 + kk, n = n0 + nn;
-        sB[nn][kk] = (k < ke && n < N) ? B[(long)k * sbk + (long)n * sbn] : (uint16_t)0;
-      }
-    }
-    __syncthreads();
-    const int r = lane & 15, kq = (lane >> 4) * 8;
-    bf16x8_t a[2], b[2];
-    for (int i = 0; i < 2; ++i) {
-      a[i] = *reinterpret_cast<const bf16x8_t*>(&sA[wm + i * 16 + r][kq]);
-      b[i] = *reinterpret_cast<const bf16x8_t*>(&sB[wn + i * 16 + r][kq]);
-    }
-    for (int i = 0; i < 2; ++i)
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    __syncthreads();
-  }
-  const int col_in = lane & 15, row_q = (lane >> 4) * 4;
-  float* const Cf = ep.Cf ? ep.Cf + (long)blockIdx.z * ep.split_stride : nullptr;
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j)
-      for (int t = 0; t < 4; ++t) {
-        const int m = m0 + wm + i * 16 + row_q + t, n = n0 + wn + j * 16 + col_in;
-        if (m >= M || n >= N) continue;
-        float v = acc[i][j][t];
-        if (ep.bias) v += ep.bias[n];
-        if (ep.bias2) v += ep.bias2[n];
-        if (ep.relu) v = fmaxf(v, 0.0f);
-        if (ep.mask && !(bf2f(ep.mask[(long)m * ep.ldm + n]) > 0.0f)) v = 0.0f;
-        if (Cf) Cf[ep.trans_f ? (long)n * ep.ldc + m : (long)m * ep.ldc + n] = v;
-        if (ep.Cb) ep.Cb[(long)m * ep.ldcb + n] = f2bf(v);
-      }
-}
-
-// C[m * ldc + n] = sum over z = 0 .. splits-1 of P[z * stride + m * N + n].
-__global__ void __launch_bounds__(256) k_split_sum(const float* __restrict__ P, int splits, long stride, int M, int N,
-                                                   float* __restrict__ C, long ldc) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long)M * N) return;
-  const int m = (int)(i / N), n = (int)(i % N);
-  float s = 0.0f;
-  for (int z = 0; z < splits; ++z) s += P[(long)z * stride + i];
-  C[(long)m * ldc + n] = s;
-}
-
 // Network input: x = pixel / 255 (pixel-major RGB), bf16.  Row r of X takes
 // pixel row idx[r] (or r when idx is null).
 __global__ void __launch_bounds__(256) k_net_input(const uint8_t* __restrict__ px, const int32_t* __restrict__ idx,
@@ -194,117 +73,487 @@ __global__ void __launch_bounds__(256) k_net_input(const uint8_t* __restrict__ p
   X[i] = f2bf((float)px[src * kNetIn + c] * (1.0f / 255.0f));
 }
 
-// Loss of a batch: -log of the probability the softmax of the 256 logits
-// gives the row's target set (tmask: 8 words, bit L of word L / 32; one bit =
-// the usual cross-entropy), plus seed_w * the mean squared error of the 6
-// seeds against the endpoints / 255.  d/dy_c of the set term is
-// p_c - [c in set] exp(y_c) / sum_{set} exp(y).  Writes dY = d(mean loss)/dY in
-// fp32 and bf16 and adds the batch's mean loss to *loss.  A wave per row (rows
-// strided over the grid); target and eps rows are taken through idx like the
-// inputs.  A row with an empty set adds only its seed term.
-__global__ void __launch_bounds__(256) k_net_loss(const float* __restrict__ Y, int n, const uint32_t* __restrict__ tmask,
-                                                  const uint8_t* __restrict__ eps, const int32_t* __restrict__ idx,
-                                                  float seed_w, float* __restrict__ dYf, uint16_t* __restrict__ dYb,
-                                                  float* __restrict__ loss) {
+// ---------------------------------------------------------------------------
+// Training step: k_train_rows, k_train_wgrad, k_grad_reduce.
+//
+// MFMA v_mfma_f32_16x16x32_bf16 operand layout (lane l of a wave): A row
+// l % 16, k (l / 16) * 8 .. +7; B column l % 16, k (l / 16) * 8 .. +7; D rows
+// (l / 16) * 4 .. +3, column l % 16.  Every operand fragment is one 16-byte
+// load of 8 consecutive k, so each operand is kept with k contiguous:
+//  - bf16 weight copies (written by the Adam step / k_train_copies; pads stay zero):
+//      W1T [256][64]  (W1 transposed, k 48..63 zero)      forward, layer 1
+//      W2T [256][256] (W2 transposed)                     forward, layer 2
+//      W3T [272][256] (W3 transposed, rows 262..271 zero)  forward, layer 3
+//      W2R [256][256] (W2 as stored)                      dH1 = dH2 W2^T
+//      W3R [256][288] (W3, columns 262..287 zero)          dH2 = dY W3^T
+//  - the batch's activations and output gradients, feature-major (row f
+//    holds feature f of every batch row, row stride ld = the batch rounded up
+//    to 32), bf16: X (64 rows, 48..63 zero), H1, H2, dY (320 rows, 262..319
+//    zero), dH2, dH1 -- the operands of the weight gradients, whose k is the
+//    batch row.
+// ---------------------------------------------------------------------------
+constexpr int kTrW1T = 0;
+constexpr int kTrW2T = kTrW1T + kNetHid * 64;
+constexpr int kTrW3T = kTrW2T + kNetHid * kNetHid;
+constexpr int kTrW2R = kTrW3T + 272 * kNetHid;
+constexpr int kTrW3R = kTrW2R + kNetHid * kNetHid;
+constexpr int kTrSize = kTrW3R + kNetHid * 288;  // bf16 elements
+constexpr int kActX = 0, kActH1 = 64, kActH2 = kActH1 + kNetHid, kActDY = kActH2 + kNetHid;
+constexpr int kActDH2 = kActDY + 320, kActDH1 = kActDH2 + kNetHid, kActRows = kActDH1 + kNetHid;  // 1408
+constexpr int kBiasCols = 2 * kNetHid + kNetOut;           // 774: db1, db2, db3 in parameter order
+constexpr long kWgPart = 64L * 256 + 256L * 256 + 256L * 320;  // floats of one split's weight gradients
+constexpr int kWgTiles = 4 + 16 + 20;                      // 64 x 64 tiles of dW1, dW2, dW3
+
+// the bf16 training copies of parameter i (a weight; biases have none)
+__device__ __forceinline__ void store_train_copies(int i, float wi, uint16_t* __restrict__ wt) {
+  const uint16_t h = f2bf(wi);
+  if (i < kOffB1) {
+    const int k = i / kNetHid, n = i % kNetHid;
+    wt[kTrW1T + n * 64 + k] = h;
+  } else if (i >= kOffW2 && i < kOffB2) {
+    const int j = i - kOffW2, k = j / kNetHid, n = j % kNetHid;
+    wt[kTrW2T + n * kNetHid + k] = h;
+    wt[kTrW2R + j] = h;
+  } else if (i >= kOffW3 && i < kOffB3) {
+    const int j = i - kOffW3, k = j / kNetOut, n = j % kNetOut;
+    wt[kTrW3T + n * kNetHid + k] = h;
+    wt[kTrW3R + k * 288 + n] = h;
+  }
+}
+
+__device__ __forceinline__ bf16x8_t frag(const uint16_t* p) { return *reinterpret_cast<const bf16x8_t*>(p); }
+
+// The weight fragments of KS k-steps (from k0) of NJ column tiles of a
+// weight copy gB (tile j: output columns col0 + cstep * j .. +15, one row of
+// gB per output column, k contiguous, row stride ldb).  nj (wave-uniform)
+// tiles are loaded.  Loading a chunk before the work that precedes its
+// MFMAs keeps the L2 round trip off the critical path.
+template <int NJ, int KS>
+struct WFrags {
+  bf16x8_t b[KS][NJ];
+};
+
+template <int NJ, int KS>
+__device__ __forceinline__ void load_frags(WFrags<NJ, KS>& f, const uint16_t* __restrict__ gB, int ldb, int col0,
+                                           int cstep, int nj, int k0, int lane) {
+  const int r = lane & 15, kq = (lane >> 4) * 8;
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+      if (j < nj) f.b[ks][j] = frag(gB + (long)(col0 + cstep * j + r) * ldb + k0 + ks * 32 + kq);
+}
+
+// acc[i][j] += A x B: A = rows 16 i .. 16 i + 15 of the LDS matrix sA (row
+// stride lda, k contiguous), k from k0 over the chunk's KS steps.
+template <int RT, int NJ, int KS>
+__device__ __forceinline__ void mfma_frags(f32x4_t (&acc)[RT][NJ], const WFrags<NJ, KS>& f, const uint16_t* sA,
+                                           int lda, int k0, int nj, int lane) {
+  const int r = lane & 15, kq = (lane >> 4) * 8;
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    bf16x8_t a[RT];
+#pragma unroll
+    for (int i = 0; i < RT; ++i) a[i] = frag(sA + (16 * i + r) * lda + k0 + ks * 32 + kq);
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+      if (j < nj)
+#pragma unroll
+        for (int i = 0; i < RT; ++i)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], f.b[ks][j], acc[i][j], 0, 0, 0);
+  }
+}
+
+template <int RT, int NJ>
+__device__ __forceinline__ void zero_acc(f32x4_t (&acc)[RT][NJ]) {
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+}
+
+// Epilogue of a 256-wide layer: wave columns c0 + 16 j + (lane % 16), rows
+// 16 i + (lane / 16) * 4 + t.  H = relu(acc + bias) (forward, mask null) or
+// acc * [mask > 0] (backward); bf16 to the LDS rows sO (when non-null) and as
+// 4 consecutive rows of one feature to gO (feature-major, row stride ld);
+// with cpo, the fp32 column sums over the workgroup's rows go to cpo[col].
+template <int RT>
+__device__ __forceinline__ void layer_out(const f32x4_t (&acc)[RT][4], int c0, int lane, const float* __restrict__ bias,
+                                          const uint16_t* mask, uint16_t* sO, uint16_t* __restrict__ gO, int ld,
+                                          float* __restrict__ cpo) {
+  constexpr int LH = kNetHid + 8;
+  const int col_in = lane & 15, rq = (lane >> 4) * 4;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int col = c0 + 16 * j + col_in;
+    const float b = bias ? bias[col] : 0.0f;
+    float cs = 0.0f;
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+      float v[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int r = 16 * i + rq + t;
+        v[t] = mask ? (bf2f(mask[r * LH + col]) > 0.0f ? acc[i][j][t] : 0.0f) : fmaxf(acc[i][j][t] + b, 0.0f);
+        cs += v[t];
+      }
+      const uint32_t p01 = pack_bf16x2(v[0], v[1]), p23 = pack_bf16x2(v[2], v[3]);
+      if (sO) {
+        const int r = 16 * i + rq;
+        sO[r * LH + col] = (uint16_t)p01;
+        sO[(r + 1) * LH + col] = (uint16_t)(p01 >> 16);
+        sO[(r + 2) * LH + col] = (uint16_t)p23;
+        sO[(r + 3) * LH + col] = (uint16_t)(p23 >> 16);
+      }
+      *reinterpret_cast<uint2*>(gO + (long)col * ld + 16 * i + rq) = make_uint2(p01, p23);
+    }
+    if (cpo) {
+      cs += __shfl_xor(cs, 16);
+      cs += __shfl_xor(cs, 32);
+      if (lane < 16) cpo[col] = cs;
+    }
+  }
+}
+
+// One workgroup (4 waves) per 16 RT batch rows: the forward pass (H1, H2,
+// logits and seeds), the loss and dY = d(mean loss)/dY, and the backward
+// products dH2 = (dY W3^T) * [H2 > 0], dH1 = (dH2 W2^T) * [H1 > 0], all
+// held in LDS; the weight-gradient operands go out feature-major (act) and
+// the bias gradients as this workgroup's column sums of dH1, dH2, dY in fp32
+// (colpart[wg][774]).  Wave w computes columns 64 w .. 64 w + 63 of each
+// 256-wide layer (RT x 4 MFMA tiles) and column tiles w, w + 4, ... of the 17
+// of the output layer.  Batch row r (r < n) takes pair idx[r] (or r); rows
+// n .. ld - 1 are zero in the inputs and gradients.  The weight fragments of
+// each layer are requested in chunks of 4 or 5 k-steps one chunk ahead of
+// their MFMAs, the backward's during the loss.
+//
+// Loss: -log of the probability the softmax of the 256 logits gives the row's
+// target set (tmask: 8 words, bit L of word L / 32; one bit = the usual
+// cross-entropy), plus seed_w * the mean squared error of the 6 seeds against
+// the endpoints / 255.  d/dy_c of the set term is p_c - [c in set] exp(y_c) /
+// sum_{set} exp(y).  A row with an empty set adds only its seed term.  A wave
+// takes its 4 RT rows at once (64 / (4 RT) lanes per row).  The batch's mean
+// loss is added to *loss (one atomic per workgroup).
+template <int RT>
+__global__ void __launch_bounds__(256) k_train_rows(int n, int ld, const uint8_t* __restrict__ px,
+                                                    const int32_t* __restrict__ idx, const uint32_t* __restrict__ tmask,
+                                                    const uint8_t* __restrict__ eps, float seed_w,
+                                                    const uint16_t* __restrict__ wt, const float* __restrict__ w,
+                                                    const float* __restrict__ base, uint16_t* __restrict__ act,
+                                                    float* __restrict__ colpart, float* __restrict__ loss) {
+  constexpr int R = 16 * RT;                                                // rows of the workgroup
+  constexpr int LX = 64 + 8, LH = kNetHid + 8, LY = 288 + 4, LD = 288 + 8;  // LDS row strides (skewed banks)
+  __shared__ __attribute__((aligned(16))) uint16_t sX[R * LX];
+  __shared__ __attribute__((aligned(16))) uint16_t sH1[R * LH];
+  __shared__ __attribute__((aligned(16))) uint16_t sH2[R * LH];
+  __shared__ __attribute__((aligned(16))) float sY[R * LY];       // logits, then dY (fp32)
+  __shared__ __attribute__((aligned(16))) uint16_t sDY[R * LD];   // dY (bf16), columns 262..287 zero
+  __shared__ __attribute__((aligned(16))) uint16_t sDH2[R * LH];  // dH2 (bf16)
   __shared__ float s_l[4];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ long s_src[R];
+  __shared__ uint32_t s_tm[R * 8];
+  __shared__ uint8_t s_eps[R * kNetSeeds];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int r0 = blockIdx.x * R, c0 = wave * 64;
+  const int col_in = lane & 15, rq = (lane >> 4) * 4;  // D layout: rows rq .. rq + 3 of a tile, column col_in
+  const int nj3 = wave == 0 ? 5 : 4;                   // output-layer column tiles of this wave
+  float* const cp = colpart + (long)blockIdx.x * kBiasCols;
+  WFrags<4, 2> fA;
+  WFrags<4, 4> fB0, fB1;
+  load_frags(fA, wt + kTrW1T, 64, c0, 16, 4, 0, lane);
+  load_frags(fB0, wt + kTrW2T, kNetHid, c0, 16, 4, 0, lane);
+  if (tid < R) s_src[tid] = r0 + tid < n ? (idx ? (long)idx[r0 + tid] : (long)(r0 + tid)) : -1;
+  __syncthreads();
+  // the rows' pixels, target sets and seeds, all requested at once; inputs
+  // x = pixel / 255 (pixel-major RGB), columns 48..63 zero
+  if (tid < R * 8) {
+    const long src = s_src[tid >> 3];
+    s_tm[tid] = src >= 0 ? tmask[src * 8 + (tid & 7)] : 0u;
+  }
+  if (tid < R * kNetSeeds) {
+    const long src = s_src[tid / kNetSeeds];
+    s_eps[tid] = src >= 0 ? eps[src * kNetSeeds + tid % kNetSeeds] : (uint8_t)0;
+  }
+  for (int e = tid; e < R * 64; e += 256) {
+    const int r = e >> 6, cc = e & 63;
+    const long src = s_src[r];
+    const float x = (src >= 0 && cc < kNetIn) ? (float)px[src * kNetIn + cc] * (1.0f / 255.0f) : 0.0f;
+    sX[r * LX + cc] = f2bf(x);
+  }
+  __syncthreads();
+  for (int e = tid; e < 64 * (R / 8); e += 256) {  // X feature-major: (feature, 8 rows)
+    const int f = e / (R / 8), rr = (e % (R / 8)) * 8;
+    uint16_t h[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) h[j] = sX[(rr + j) * LX + f];
+    uint4 v;
+    __builtin_memcpy(&v, h, 16);
+    *reinterpret_cast<uint4*>(act + (long)(kActX + f) * ld + r0 + rr) = v;
+  }
+  {  // layer 1: H1 = relu(X W1 + b1)
+    f32x4_t acc[RT][4];
+    zero_acc(acc);
+    mfma_frags(acc, fA, sX, LX, 0, 4, lane);
+    load_frags(fB1, wt + kTrW2T, kNetHid, c0, 16, 4, 128, lane);
+    layer_out<RT>(acc, c0, lane, w + kOffB1, nullptr, sH1, act + (long)kActH1 * ld + r0, ld, nullptr);
+  }
+  __syncthreads();
+  WFrags<5, 4> fC0, fC1;
+  {  // layer 2: H2 = relu(H1 W2 + b2)
+    f32x4_t acc[RT][4];
+    zero_acc(acc);
+    mfma_frags(acc, fB0, sH1, LH, 0, 4, lane);
+    load_frags(fC0, wt + kTrW3T, kNetHid, 16 * wave, 64, nj3, 0, lane);
+    mfma_frags(acc, fB1, sH1, LH, 128, 4, lane);
+    load_frags(fC1, wt + kTrW3T, kNetHid, 16 * wave, 64, nj3, 128, lane);
+    layer_out<RT>(acc, c0, lane, w + kOffB2, nullptr, sH2, act + (long)kActH2 * ld + r0, ld, nullptr);
+  }
+  __syncthreads();
+  WFrags<4, 4> fD0;
+  WFrags<4, 5> fD1;
+  {  // layer 3: Y = H2 W3 + b3 (+ base), column tiles wave, wave + 4, ... of 17
+    f32x4_t acc[RT][5];
+    zero_acc(acc);
+    mfma_frags(acc, fC0, sH2, LH, 0, nj3, lane);
+    load_frags(fD0, wt + kTrW3R, 288, c0, 16, 4, 0, lane);
+    mfma_frags(acc, fC1, sH2, LH, 128, nj3, lane);
+    load_frags(fD1, wt + kTrW3R, 288, c0, 16, 4, 128, lane);
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+      if (j < nj3) {
+        const int col = 16 * (wave + 4 * j) + col_in;
+        const float b = col < kNetOut ? w[kOffB3 + col] + (base ? base[col] : 0.0f) : 0.0f;
+#pragma unroll
+        for (int i = 0; i < RT; ++i)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) sY[(16 * i + rq + t) * LY + col] = col < kNetOut ? acc[i][j][t] + b : 0.0f;
+      }
+  }
+  __syncthreads();
   const float inv_n = 1.0f / (float)n;
   float l = 0.0f;  // this lane's share of the wave's rows' losses
-  for (int row = blockIdx.x * 4 + wave; row < n; row += gridDim.x * 4) {
-    const long src = idx ? (long)idx[row] : row;
-    const float* y = Y + (size_t)row * kNetOut;
+  {  // loss and dY: the wave's 4 RT rows at once, LPR lanes per row
+    constexpr int LPR = 16 / RT, CPL = kNetLogits / LPR;
+    const int r = wave * 4 * RT + lane / LPR, sub = lane % LPR;
+    const bool valid = r0 + r < n;
+    float* y = sY + r * LY;
+    uint16_t* dyb = sDY + r * LD;
     uint32_t tm[8];
 #pragma unroll
-    for (int w = 0; w < 8; ++w) tm[w] = tmask[src * 8 + w];
+    for (int q = 0; q < 8; ++q) tm[q] = s_tm[r * 8 + q];
+    float e[CPL];
     float mx = -3.0e38f;
-    for (int c = lane; c < kNetLogits; c += 64) mx = fmaxf(mx, y[c]);
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    float se = 0.0f, ss = 0.0f;  // sum over all logits, over the set
-    for (int c = lane; c < kNetLogits; c += 64) {
-      const float e = expf(y[c] - mx);
-      se += e;
-      if ((tm[c >> 5] >> (c & 31)) & 1u) ss += e;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      e[q] = y[sub + LPR * q];
+      mx = fmaxf(mx, e[q]);
     }
-    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    float se = 0.0f, ss = 0.0f;  // sum over all logits, over the set
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const int cc = sub + LPR * q;
+      e[q] = expf(e[q] - mx);
+      se += e[q];
+      if ((tm[cc >> 5] >> (cc & 31)) & 1u) ss += e[q];
+    }
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) {
       se += __shfl_xor(se, o);
       ss += __shfl_xor(ss, o);
     }
-    const float lse = logf(se);
     const bool has_set = ss > 0.0f;
-    if (has_set && lane == 0) l += lse - logf(ss);
+    if (valid && has_set && sub == 0) l += logf(se) - logf(ss);
+    const float gs = valid && has_set ? inv_n / se : 0.0f, gq = valid && has_set ? inv_n / ss : 0.0f;
 #pragma unroll
-    for (int j = 0; j < 5; ++j) {
-      const int c = lane + 64 * j;
-      if (c >= kNetOut) break;
-      float g;
-      if (c < kNetLogits) {
-        const float e = expf(y[c] - mx);
-        const float q = (has_set && ((tm[c >> 5] >> (c & 31)) & 1u)) ? e / ss : 0.0f;
-        g = has_set ? (e / se - q) * inv_n : 0.0f;
-      } else {
-        const float t = (float)eps[src * kNetSeeds + (c - kNetLogits)] * (1.0f / 255.0f);
-        const float d = y[c] - t;
+    for (int q = 0; q < CPL; ++q) {
+      const int cc = sub + LPR * q;
+      const float g = e[q] * gs - (((tm[cc >> 5] >> (cc & 31)) & 1u) ? e[q] * gq : 0.0f);
+      y[cc] = g;
+      dyb[cc] = f2bf(g);
+    }
+    for (int cc = kNetLogits + sub; cc < 288; cc += LPR) {  // the seeds, then zero columns
+      float g = 0.0f;
+      if (valid && cc < kNetOut) {
+        const float t = (float)s_eps[r * kNetSeeds + (cc - kNetLogits)] * (1.0f / 255.0f);
+        const float d = y[cc] - t;
         l += seed_w * d * d * (1.0f / kNetSeeds);
         g = 2.0f * seed_w * d * (1.0f / kNetSeeds) * inv_n;
       }
-      dYf[(size_t)row * kNetOut + c] = g;
-      dYb[(size_t)row * kNetOut + c] = f2bf(g);
+      y[cc] = g;
+      dyb[cc] = f2bf(g);
     }
   }
-  // one atomic per workgroup: same-address atomics serialise
   for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
   if (lane == 0) s_l[wave] = l;
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(loss, (s_l[0] + s_l[1] + s_l[2] + s_l[3]) * inv_n);
+  if (tid == 0) atomicAdd(loss, (s_l[0] + s_l[1] + s_l[2] + s_l[3]) * inv_n);
+  // db3 column sums (fp32, rows in order); dY feature-major (rows 262..319 zero)
+  for (int cc = tid; cc < kNetOut; cc += 256) {
+    float s = 0.0f;
+    for (int r = 0; r < R; ++r) s += sY[r * LY + cc];
+    cp[2 * kNetHid + cc] = s;
+  }
+  for (int e = tid; e < 320 * (R / 8); e += 256) {
+    const int f = e / (R / 8), rr = (e % (R / 8)) * 8;
+    uint16_t h[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) h[j] = f < 288 ? sDY[(rr + j) * LD + f] : (uint16_t)0;
+    uint4 v;
+    __builtin_memcpy(&v, h, 16);
+    *reinterpret_cast<uint4*>(act + (long)(kActDY + f) * ld + r0 + rr) = v;
+  }
+  WFrags<4, 4> fE0, fE1;
+  {  // dH2 = (dY W3^T) * [H2 > 0]
+    f32x4_t acc[RT][4];
+    zero_acc(acc);
+    mfma_frags(acc, fD0, sDY, LD, 0, 4, lane);
+    load_frags(fE0, wt + kTrW2R, kNetHid, c0, 16, 4, 0, lane);
+    mfma_frags(acc, fD1, sDY, LD, 128, 4, lane);
+    load_frags(fE1, wt + kTrW2R, kNetHid, c0, 16, 4, 128, lane);
+    layer_out<RT>(acc, c0, lane, nullptr, sH2, sDH2, act + (long)kActDH2 * ld + r0, ld, cp + kNetHid);
+  }
+  __syncthreads();
+  {  // dH1 = (dH2 W2^T) * [H1 > 0]
+    f32x4_t acc[RT][4];
+    zero_acc(acc);
+    mfma_frags(acc, fE0, sDH2, LH, 0, 4, lane);
+    mfma_frags(acc, fE1, sDH2, LH, 128, 4, lane);
+    layer_out<RT>(acc, c0, lane, nullptr, sH1, nullptr, act + (long)kActDH1 * ld + r0, ld, cp);
+  }
 }
 
-// Column sums in two deterministic stages: part[p][c] = sum of rows
-// p*kColsumRows .. (p+1)*kColsumRows - 1, then out[c] = sum over p in a fixed
-// order (k_colsum_final).
-constexpr int kColsumRows = 32;  // a 4096-row batch: 128 partials per column
-__global__ void __launch_bounds__(256) k_colsum_part(const float* __restrict__ X, int rows, int cols, long ld,
+// Weight gradients dW1 = X^T dH1, dW2 = H1^T dH2, dW3 = H2^T dY as 40 tiles
+// of 64 x 64 (blockIdx.x) with the batch split into gridDim.y ranges of kspan
+// rows (a multiple of 32); split s stores its partial products to part + s *
+// kWgPart ([64][256], [256][256], [256][320]; k_grad_reduce adds them in a
+// fixed order).  Wave w owns the 32 x 32 quadrant (w / 2, w % 2) as 2 x 2
+// MFMA tiles; both operands are feature-major rows of act, loaded straight
+// into registers.
+__global__ void __launch_bounds__(256) k_train_wgrad(const uint16_t* __restrict__ act, int ld, int kspan,
                                                      float* __restrict__ part) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x, p = blockIdx.y;
-  if (c >= cols) return;
-  float s = 0.0f;
-  const int r1 = min(rows, (p + 1) * kColsumRows);
-  for (int r = p * kColsumRows; r < r1; ++r) s += X[(long)r * ld + c];
-  part[(long)p * cols + c] = s;
+  const int t = blockIdx.x, s = blockIdx.y;
+  int arow, brow, m0, n0, ncols;
+  long poff;
+  if (t < 4) {
+    arow = kActX, brow = kActDH1, m0 = 0, n0 = t * 64, ncols = kNetHid, poff = 0;
+  } else if (t < 20) {
+    const int u = t - 4;
+    arow = kActH1, brow = kActDH2, m0 = (u >> 2) * 64, n0 = (u & 3) * 64, ncols = kNetHid, poff = 64L * 256;
+  } else {
+    const int u = t - 20;
+    arow = kActH2, brow = kActDY, m0 = (u / 5) * 64, n0 = (u % 5) * 64, ncols = 320, poff = 64L * 256 + 256L * 256;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, r = lane & 15, kq = (lane >> 4) * 8;
+  const uint16_t* A0 = act + (long)(arow + m0 + wm + r) * ld + kq;
+  const uint16_t* B0 = act + (long)(brow + n0 + wn + r) * ld + kq;
+  const long l16 = 16L * ld;
+  f32x4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const int kb = s * kspan, ke = min(ld, kb + kspan);
+  int k = kb;
+  for (; k + 256 <= ke; k += 256) {  // 8 steps' fragments requested before their MFMAs
+    bf16x8_t a[8][2], b[8][2];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      a[q][0] = frag(A0 + k + 32 * q);
+      a[q][1] = frag(A0 + l16 + k + 32 * q);
+      b[q][0] = frag(B0 + k + 32 * q);
+      b[q][1] = frag(B0 + l16 + k + 32 * q);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q][i], b[q][j], acc[i][j], 0, 0, 0);
+  }
+  for (; k < ke; k += 32) {
+    const bf16x8_t a0 = frag(A0 + k), a1 = frag(A0 + l16 + k), b0 = frag(B0 + k), b1 = frag(B0 + l16 + k);
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+  }
+  float* P = part + (long)s * kWgPart + poff;
+  const int col_in = lane & 15, rq = (lane >> 4) * 4;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        P[(long)(m0 + wm + 16 * i + rq + q) * ncols + n0 + wn + 16 * j + col_in] = acc[i][j][q];
 }
 
-// out[c] = sum over p of part[p][c]: a wave per column; lane l adds parts
-// l, l + 64, ... in order, then the wave adds its lanes in a fixed tree (the
-// same order every run).
-__global__ void __launch_bounds__(256) k_colsum_final(const float* __restrict__ part, int parts, int cols,
-                                                      float* __restrict__ out) {
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (c >= cols) return;  // wave-uniform
-  float s = 0.0f;
-  for (int p = lane; p < parts; p += 64) s += part[(long)p * cols + c];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (lane == 0) out[c] = s;
-}
+// One Adam step (on when lr > 0); c1, c2 = 1 - beta1^t, 1 - beta2^t.
+struct AdamStep {
+  float lr, b1, b2, c1, c2, eps;
+};
 
-// Adam on the fp32 master parameters; refreshes their bf16 copy.
-// c1, c2 = 1 - beta1^t, 1 - beta2^t.
-__global__ void __launch_bounds__(256) k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
-                                              float* __restrict__ v, uint16_t* __restrict__ wb, int n, float lr,
-                                              float b1, float b2, float c1, float c2, float epsv) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.0f - b1) * gi;
-  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+// Adam on fp32 master parameter i with gradient gi; refreshes its bf16 copies.
+__device__ __forceinline__ void adam_update(int i, float gi, float* __restrict__ w, float* __restrict__ m,
+                                            float* __restrict__ v, uint16_t* __restrict__ wt, const AdamStep& a) {
+  const float mi = a.b1 * m[i] + (1.0f - a.b1) * gi;
+  const float vi = a.b2 * v[i] + (1.0f - a.b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
-  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + epsv);
+  const float wi = w[i] - a.lr * (mi / a.c1) / (sqrtf(vi / a.c2) + a.eps);
   w[i] = wi;
-  wb[i] = f2bf(wi);
+  store_train_copies(i, wi, wt);
 }
 
-__global__ void __launch_bounds__(256) k_to_bf16(const float* __restrict__ x, int n, uint16_t* __restrict__ y) {
+// g = d(mean loss)/d(parameters), then (a.lr > 0) the Adam step: a weight's
+// gradient is the sum of the splits' partials in split order; a bias's the
+// sum of the k_train_rows workgroups' column sums (a wave per bias: lane l
+// adds workgroups l, l + 64, ... in order, then the lanes in a fixed tree).
+// Blocks 0 .. nwb - 1 take the parameters by index, the others the 774
+// biases four at a time.
+__global__ void __launch_bounds__(256) k_grad_reduce(const float* __restrict__ part, int splits,
+                                                     const float* __restrict__ colpart, int nwg, int nwb,
+                                                     float* __restrict__ g, float* __restrict__ w,
+                                                     float* __restrict__ m, float* __restrict__ v,
+                                                     uint16_t* __restrict__ wt, AdamStep a) {
+  if ((int)blockIdx.x < nwb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    long p;
+    if (i < kOffB1)
+      p = i;
+    else if (i >= kOffW2 && i < kOffB2)
+      p = 64L * 256 + (i - kOffW2);
+    else if (i >= kOffW3 && i < kOffB3)
+      p = 64L * 256 + 256L * 256 + (long)((i - kOffW3) / kNetOut) * 320 + (i - kOffW3) % kNetOut;
+    else
+      return;  // a bias, or past the end
+    float s = 0.0f;
+#pragma unroll 4
+    for (int z = 0; z < splits; ++z) s += part[(long)z * kWgPart + p];
+    g[i] = s;
+    if (a.lr > 0.0f) adam_update(i, s, w, m, v, wt, a);
+    return;
+  }
+  const int bc = ((int)blockIdx.x - nwb) * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (bc >= kBiasCols) return;  // wave-uniform
+  float s = 0.0f;
+  for (int q = lane; q < nwg; q += 64) s += colpart[(long)q * kBiasCols + bc];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) {
+    const int i = bc < kNetHid ? kOffB1 + bc : bc < 2 * kNetHid ? kOffB2 + bc - kNetHid : kOffB3 + bc - 2 * kNetHid;
+    g[i] = s;
+    if (a.lr > 0.0f) adam_update(i, s, w, m, v, wt, a);
+  }
+}
+
+// the bf16 training copies of all n parameters (after they are set)
+__global__ void __launch_bounds__(256) k_train_copies(const float* __restrict__ w, int n, uint16_t* __restrict__ wt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = f2bf(x[i]);
+  if (i < n) store_train_copies(i, w[i], wt);
 }
 
 // Training pairs of the last encoded step: every block whose search window

@@ -142,7 +142,7 @@ struct GraphSet {
 // their bf16 copy, Adam moments and gradient.
 struct NetState {
   float* w = nullptr;
-  uint16_t* wb = nullptr;
+  uint16_t* wt = nullptr;  // bf16 training copies (kTrSize, pads zero)
   float *m = nullptr, *v = nullptr, *g = nullptr;
   int64_t t = 0;           // Adam steps taken
   bool ready = false;      // parameters set
@@ -155,15 +155,18 @@ struct NetState {
   uint16_t* pk = nullptr;  // packed inference image (net_infer.h)
 };
 
-// Activations of a forward (and backward) pass over `rows` blocks.
+// Network workspace over `rows` blocks: the inputs of a forward pass (X) and,
+// for training, the batch's feature-major activations and gradients
+// (kActRows x ld bf16, ld = rows rounded up to 32), the k_train_rows
+// workgroups' bias column sums and the weight-gradient partials of up to
+// kMaxSplits batch splits.
 struct NetWork {
   int rows = 0;
   bool train = false;
-  uint16_t *X = nullptr, *H1 = nullptr, *H2 = nullptr, *dYb = nullptr, *dH2b = nullptr, *dH1b = nullptr;
-  float *Y = nullptr, *dYf = nullptr, *dH2f = nullptr, *dH1f = nullptr, *part = nullptr;
-  float* split = nullptr;  // split-K partial products of the weight gradients (kSplitFloats)
+  uint16_t *X = nullptr, *act = nullptr;
+  float *colpart = nullptr, *split = nullptr;
   void release() {
-    void* b[] = {X, H1, H2, dYb, dH2b, dH1b, Y, dYf, dH2f, dH1f, part, split};
+    void* b[] = {X, act, colpart, split};
     for (void* p : b)
       if (p) (void)hipFree(p);
     *this = NetWork();
@@ -337,170 +340,60 @@ inline unsigned grid_for(int n, int wg) { return (unsigned)std::max(1, (n + wg -
 int net_alloc(NetState& n) {
   if (n.w) return MPTC_OK;
   int rc;
-  if ((rc = dalloc(&n.w, kNetParams)) || (rc = dalloc(&n.wb, kNetParams)) || (rc = dalloc(&n.m, kNetParams)) ||
+  if ((rc = dalloc(&n.w, kNetParams)) || (rc = dalloc(&n.wt, kTrSize)) || (rc = dalloc(&n.m, kNetParams)) ||
       (rc = dalloc(&n.v, kNetParams)) || (rc = dalloc(&n.g, kNetParams)) || (rc = dalloc(&n.prior, kNetLogits)) ||
       (rc = dalloc(&n.pk, kPkSize)) || (rc = dalloc(&n.base, kNetOut)) || (rc = dalloc(&n.sim, kSimBins)))
     return rc;
-  if (hipMemset(n.base, 0, sizeof(float) * kNetOut) != hipSuccess) return fail(MPTC_EHIP, "hipMemset failed");
+  if (hipMemset(n.base, 0, sizeof(float) * kNetOut) != hipSuccess ||
+      hipMemset(n.wt, 0, sizeof(uint16_t) * kTrSize) != hipSuccess)  // the copies' pads stay zero
+    return fail(MPTC_EHIP, "hipMemset failed");
   return MPTC_OK;
 }
 
 void net_free(NetState& n) {
-  void* b[] = {n.w, n.wb, n.m, n.v, n.g, n.prior, n.pk, n.base, n.sim};
+  void* b[] = {n.w, n.wt, n.m, n.v, n.g, n.prior, n.pk, n.base, n.sim};
   for (void* p : b)
     if (p) (void)hipFree(p);
   n = NetState();
 }
 
-// Split-K weight gradients (gemm_wgrad): about kSplitGrid workgroups, so the
-// partials (splits x M x N) fit kSplitGrid tiles of 64 x 64.
-constexpr int kSplitGrid = 256;
-constexpr size_t kSplitFloats = (size_t)kSplitGrid * 64 * 64;
+constexpr int kMaxSplits = 16;  // batch splits of the weight gradients
+constexpr int kTrainRT = 1;     // k_train_rows: 16 x kTrainRT batch rows per workgroup
 
 int work_ensure(NetWork& w, int rows, bool train) {
   if (w.rows >= rows && (w.train || !train)) return MPTC_OK;
   w.release();
-  const size_t r = (size_t)rows;
+  const size_t r = (size_t)rows, ld = (r + 31) / 32 * 32;
   int rc;
-  if ((rc = dalloc(&w.X, r * kNetIn)) || (rc = dalloc(&w.H1, r * kNetHid)) || (rc = dalloc(&w.H2, r * kNetHid)))
+  if ((rc = dalloc(&w.X, r * kNetIn))) return rc;
+  if (train && ((rc = dalloc(&w.act, ld * kActRows)) || (rc = dalloc(&w.colpart, ld / 16 * kBiasCols)) ||
+                (rc = dalloc(&w.split, (size_t)kMaxSplits * kWgPart))))
     return rc;
-  if (train) {
-    const size_t parts = (r + kColsumRows - 1) / kColsumRows;
-    if ((rc = dalloc(&w.Y, r * kNetOut)) || (rc = dalloc(&w.dYf, r * kNetOut)) || (rc = dalloc(&w.dYb, r * kNetOut)) ||
-        (rc = dalloc(&w.dH2f, r * kNetHid)) || (rc = dalloc(&w.dH2b, r * kNetHid)) ||
-        (rc = dalloc(&w.dH1f, r * kNetHid)) || (rc = dalloc(&w.dH1b, r * kNetHid)) ||
-        (rc = dalloc(&w.part, parts * kNetOut)) || (rc = dalloc(&w.split, kSplitFloats)))
-      return rc;
-  }
   w.rows = rows;
   w.train = train;
   return MPTC_OK;
 }
 
-GemmEpi epi() {
-  GemmEpi e;
-  std::memset(&e, 0, sizeof(e));
-  return e;
-}
-
-// C = A * B (strided bf16 views) with the epilogue `ep`; rows are split over
-// launches so that the grid's y dimension stays within its limit.
-int gemm(mptc_ctx* c, hipStream_t s, int kid, int M, int N, int K, const uint16_t* A, long sam, long sak,
-         const uint16_t* B, long sbk, long sbn, GemmEpi ep) {
-  constexpr int kRowsPerLaunch = 65535 * 64;
-  for (int m0 = 0; m0 < M; m0 += kRowsPerLaunch) {
-    const int mc = std::min(M - m0, kRowsPerLaunch);
-    GemmEpi e = ep;
-    if (e.mask) e.mask += (long)m0 * e.ldm;
-    if (e.Cf) e.Cf += e.trans_f ? (long)m0 : (long)m0 * e.ldc;
-    if (e.Cb) e.Cb += (long)m0 * e.ldcb;
-    const uint16_t* a = A + (long)m0 * sam;
-    const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((mc + 63) / 64));
-    int rc = launch(c, s, kid, [&] {
-      hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, mc, N, K, a, sam, sak, B, sbk, sbn, e);
-    });
-    if (rc) return rc;
-  }
-  return MPTC_OK;
-}
-
-// Weight gradient C (M x N, row stride ldc) = A * B with K = the batch rows:
-// few output tiles and a long K, so K is split over about kSplitGrid
-// workgroups and the partials summed in a fixed order (k_split_sum).
-int gemm_wgrad(mptc_ctx* c, hipStream_t s, NetWork& w, int M, int N, int K, const uint16_t* A, long sam, long sak,
-               const uint16_t* B, long sbk, long sbn, float* C, long ldc) {
-  const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-  const int splits = std::max(1, std::min(kSplitGrid / tiles, (K + 63) / 64));
-  GemmEpi e = epi();
-  if (splits == 1 || (size_t)splits * M * N > kSplitFloats) {  // enough tiles already: no split
-    e.Cf = C;
-    e.ldc = ldc;
-    return gemm(c, s, KF_NET_FWD, M, N, K, A, sam, sak, B, sbk, sbn, e);
-  }
-  e.Cf = w.split;
-  e.ldc = N;
-  e.split_stride = (long)M * N;
-  const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64), (unsigned)splits);
+// The gradients of one training step over `rows` batch rows (pair idx[r], or
+// r): n.g = d(mean loss)/d(parameters), the mean loss added to *loss (device),
+// and, when adam.lr > 0, the Adam step on n.  Three launches: the rows'
+// forward, loss and backward products (k_train_rows), the weight gradients
+// over about 256-row batch splits (k_train_wgrad), and their fixed-order sums
+// with the bias sums and the Adam step (k_grad_reduce).
+int net_grad_q(mptc_ctx* c, hipStream_t s, NetState& n, NetWork& w, int rows, const uint8_t* px,
+               const int32_t* idx, const uint32_t* target, const uint8_t* eps, float seed_w, float* loss,
+               const AdamStep& adam) {
+  const int ld = (rows + 31) / 32 * 32, nwg = ld / (16 * kTrainRT);
+  const int splits = std::max(1, std::min(kMaxSplits, ld / 256));
+  const int kspan = ((ld + splits - 1) / splits + 31) / 32 * 32;
+  const int nwb = (int)grid_for(kNetParams, 256);
   return launch(c, s, KF_NET_FWD, [&] {
-    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, e);
-    hipLaunchKernelGGL(k_split_sum, dim3(grid_for(M * N, 256)), dim3(256), 0, s, w.split, splits, (long)M * N, M, N,
-                       C, ldc);
+    hipLaunchKernelGGL(k_train_rows<kTrainRT>, dim3(nwg), dim3(256), 0, s, rows, ld, px, idx, target, eps, seed_w, n.wt, n.w,
+                       n.use_base ? n.base : nullptr, w.act, w.colpart, loss);
+    hipLaunchKernelGGL(k_train_wgrad, dim3(kWgTiles, splits), dim3(256), 0, s, w.act, ld, kspan, w.split);
+    hipLaunchKernelGGL(k_grad_reduce, dim3(nwb + (kBiasCols + 3) / 4), dim3(256), 0, s, w.split, splits,
+                       w.colpart, nwg, nwb, n.g, n.w, n.m, n.v, n.wt, adam);
   });
-}
-
-// Forward pass over w.X (rows blocks): H1, H2 in w, outputs to y (row-major
-// rows x kNetOut, or transposed kNetOut x rows when trans).
-int net_forward_q(mptc_ctx* c, hipStream_t s, const NetState& n, NetWork& w, int rows, float* y, bool trans) {
-  int rc;
-  GemmEpi e = epi();
-  e.bias = n.w + kOffB1;
-  e.relu = 1;
-  e.Cb = w.H1;
-  e.ldcb = kNetHid;
-  if ((rc = gemm(c, s, KF_NET_FWD, rows, kNetHid, kNetIn, w.X, kNetIn, 1, n.wb + kOffW1, kNetHid, 1, e))) return rc;
-  e.bias = n.w + kOffB2;
-  e.Cb = w.H2;
-  if ((rc = gemm(c, s, KF_NET_FWD, rows, kNetHid, kNetHid, w.H1, kNetHid, 1, n.wb + kOffW2, kNetHid, 1, e)))
-    return rc;
-  e = epi();
-  e.bias = n.w + kOffB3;
-  e.bias2 = n.use_base ? n.base : nullptr;
-  e.Cf = y;
-  e.trans_f = trans ? 1 : 0;
-  e.ldc = trans ? rows : kNetOut;
-  return gemm(c, s, KF_NET_FWD, rows, kNetOut, kNetHid, w.H2, kNetHid, 1, n.wb + kOffW3, kNetOut, 1, e);
-}
-
-int colsum_q(mptc_ctx* c, hipStream_t s, const float* X, int rows, int cols, long ld, float* part, float* out) {
-  const int parts = (rows + kColsumRows - 1) / kColsumRows;
-  return launch(c, s, KF_NET_FWD, [&] {
-    hipLaunchKernelGGL(k_colsum_part, dim3(grid_for(cols, 256), parts), dim3(256), 0, s, X, rows, cols, ld, part);
-    hipLaunchKernelGGL(k_colsum_final, dim3(grid_for(cols, 4)), dim3(256), 0, s, part, parts, cols, out);  // a wave per column
-  });
-}
-
-// Loss and gradient of a forward pass over w (rows = the batch): n.g receives
-// d(mean loss)/d(parameters), *loss (device) the mean loss.
-constexpr int kLossGrid = 256;  // workgroups of k_net_loss (one loss atomic each)
-int net_backward_q(mptc_ctx* c, hipStream_t s, NetState& n, NetWork& w, int rows, const uint32_t* target,
-                   const uint8_t* eps, const int32_t* idx, float seed_w, float* loss) {
-  int rc;
-  if ((rc = launch(c, s, KF_NET_FWD, [&] {
-         hipLaunchKernelGGL(k_net_loss, dim3(std::min(grid_for(rows, 4), (unsigned)kLossGrid)), dim3(256), 0, s, w.Y, rows, target,
-                            eps, idx, seed_w, w.dYf, w.dYb, loss);
-       })))
-    return rc;
-  GemmEpi e = epi();
-  // layer 3: dW3 = H2^T dY, db3 = sum dY, dH2 = (dY W3^T) * [H2 > 0]
-  if ((rc = gemm_wgrad(c, s, w, kNetHid, kNetOut, rows, w.H2, 1, kNetHid, w.dYb, kNetOut, 1, n.g + kOffW3, kNetOut)))
-    return rc;
-  if ((rc = colsum_q(c, s, w.dYf, rows, kNetOut, kNetOut, w.part, n.g + kOffB3))) return rc;
-  e = epi();
-  e.mask = w.H2;
-  e.ldm = kNetHid;
-  e.Cf = w.dH2f;
-  e.ldc = kNetHid;
-  e.Cb = w.dH2b;
-  e.ldcb = kNetHid;
-  if ((rc = gemm(c, s, KF_NET_FWD, rows, kNetHid, kNetOut, w.dYb, kNetOut, 1, n.wb + kOffW3, 1, kNetOut, e)))
-    return rc;
-  // layer 2
-  if ((rc = gemm_wgrad(c, s, w, kNetHid, kNetHid, rows, w.H1, 1, kNetHid, w.dH2b, kNetHid, 1, n.g + kOffW2, kNetHid)))
-    return rc;
-  if ((rc = colsum_q(c, s, w.dH2f, rows, kNetHid, kNetHid, w.part, n.g + kOffB2))) return rc;
-  e = epi();
-  e.mask = w.H1;
-  e.ldm = kNetHid;
-  e.Cf = w.dH1f;
-  e.ldc = kNetHid;
-  e.Cb = w.dH1b;
-  e.ldcb = kNetHid;
-  if ((rc = gemm(c, s, KF_NET_FWD, rows, kNetHid, kNetHid, w.dH2b, kNetHid, 1, n.wb + kOffW2, 1, kNetHid, e)))
-    return rc;
-  // layer 1
-  if ((rc = gemm_wgrad(c, s, w, kNetIn, kNetHid, rows, w.X, 1, kNetIn, w.dH1b, kNetHid, 1, n.g + kOffW1, kNetHid)))
-    return rc;
-  return colsum_q(c, s, w.dH1f, rows, kNetHid, kNetHid, w.part, n.g + kOffB1);
 }
 
 int net_input_q(mptc_ctx* c, hipStream_t s, const uint8_t* px, const int32_t* idx, int rows, uint16_t* X) {
@@ -1292,7 +1185,7 @@ static int check_net(mptc_ctx* c, int net) {
 static int upload_params(mptc_ctx* c, NetState& n, const float* params) {
   hipStream_t s = c->stream;
   HIP_TRY(hipMemcpyAsync(n.w, params, sizeof(float) * kNetParams, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(kNetParams, 256)), dim3(256), 0, s, n.w, kNetParams, n.wb);
+  hipLaunchKernelGGL(k_train_copies, dim3(grid_for(kNetParams, 256)), dim3(256), 0, s, n.w, kNetParams, n.wt);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemsetAsync(n.m, 0, sizeof(float) * kNetParams, s));
   HIP_TRY(hipMemsetAsync(n.v, 0, sizeof(float) * kNetParams, s));
@@ -1368,9 +1261,8 @@ int32_t mptc_net_gradients(mptc_ctx* c, int32_t net, const uint8_t* d_px, const 
   float* d_loss = nullptr;
   if ((rc = dalloc(&d_loss, 1))) return rc;
   HIP_TRY(hipMemsetAsync(d_loss, 0, sizeof(float), s));
-  if ((rc = work_ensure(c->nw, n, true)) || (rc = net_input_q(c, s, d_px, nullptr, n, c->nw.X)) ||
-      (rc = net_forward_q(c, s, ns, c->nw, n, c->nw.Y, false)) ||
-      (rc = net_backward_q(c, s, ns, c->nw, n, d_target, d_eps, nullptr, seed_w, d_loss))) {
+  if ((rc = work_ensure(c->nw, n, true)) ||
+      (rc = net_grad_q(c, s, ns, c->nw, n, d_px, nullptr, d_target, d_eps, seed_w, d_loss, AdamStep{}))) {
     (void)hipFree(d_loss);
     return rc;
   }
@@ -1426,21 +1318,17 @@ int32_t mptc_net_train(mptc_ctx* c, int32_t net, const uint8_t* d_px, const uint
     }
     const int32_t* idx = d_perm + pos;
     pos += batch;
-    if ((rc = net_input_q(c, s, d_px, idx, batch, c->nw.X)) ||
-        (rc = net_forward_q(c, s, ns, c->nw, batch, c->nw.Y, false)) ||
-        (rc = net_backward_q(c, s, ns, c->nw, batch, d_target, d_eps, idx, cfg->seed_weight, d_losses + step)))
-      return done(rc);
     float lr = cfg->lr;
     if (cfg->lr_final > 0.0f && cfg->steps > 1) {
       const double u = (double)step / (double)(cfg->steps - 1);
       lr = (float)(cfg->lr_final + 0.5 * (cfg->lr - cfg->lr_final) * (1.0 + std::cos(3.141592653589793 * u)));
     }
     ns.t += 1;
-    const float c1 = (float)(1.0 - std::pow((double)b1, (double)ns.t));
-    const float c2 = (float)(1.0 - std::pow((double)b2, (double)ns.t));
-    hipLaunchKernelGGL(k_adam, dim3(grid_for(kNetParams, 256)), dim3(256), 0, s, ns.w, ns.g, ns.m, ns.v, ns.wb,
-                       kNetParams, lr, b1, b2, c1, c2, epsv);
-    if (hipGetLastError() != hipSuccess) return done(fail(MPTC_EHIP, "k_adam launch failed"));
+    const AdamStep adam{lr, b1, b2, (float)(1.0 - std::pow((double)b1, (double)ns.t)),
+                        (float)(1.0 - std::pow((double)b2, (double)ns.t)), epsv};
+    if ((rc = net_grad_q(c, s, ns, c->nw, batch, d_px, idx, d_target, d_eps, cfg->seed_weight, d_losses + step,
+                         adam)))
+      return done(rc);
   }
   hipLaunchKernelGGL(k_pack_infer, dim3(grid_for(kL3Rows * kNetHid, 256)), dim3(256), 0, s, ns.w,
                      ns.use_base ? ns.base : nullptr, ns.pk);
