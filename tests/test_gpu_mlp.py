@@ -1,4 +1,5 @@
-"""The proposal network's GPU kernels (BF16 MFMA GEMMs, loss, bias sums,
+"""The proposal network's GPU kernels (inference; the fused training step:
+BF16 MFMA forward, loss and backward products, weight gradients, bias sums,
 Adam) against the CPU restatement in oracle/mlp.py."""
 from __future__ import annotations
 
