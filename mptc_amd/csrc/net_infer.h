@@ -367,49 +367,55 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_net_l3(const uint4* __rest
       if (act) admitted_words(win_mask(net, S, fbx, fby, dp->bw, dp->bh), vw);
     }
     if (NT == 8 && sim) {
-      // Index similarity: each admitted offset's score gains sim[number of
-      // pixels whose index differs between the own word and the candidate's].
-      // When the tile's 32 blocks lie in one row of one frame, their windows'
-      // words are staged in LDS once (coalesced loads); otherwise each lane
-      // reads its candidates' words from memory.
-      const int g0 = tile * 32;
-      bool staged = false;
-      int bx_lo = 0;
-      if (g0 + 31 < rows) {
-        const int f0 = sid[g0];
-        const FrameDev& d0 = fr[f0];
-        const int k0 = g0 - d0.base;
-        if (sid[g0 + 31] == f0 && k0 / d0.bw == (k0 + 31) / d0.bw) {
-          staged = true;
-          bx_lo = k0 % d0.bw;
-          const int by0 = k0 / d0.bw;
-          uint32_t* wv = s_win[threadIdx.x >> 6];
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the last tile's reads are done
-          __builtin_amdgcn_wave_barrier();
-          for (int idx = lane; idx < kNetWin * kWinCols; idx += 64) {
-            const int r = idx / kWinCols, col = idx - r * kWinCols;
-            const int y = net == 0 ? by0 + r - 8 : by0 - r, x = bx_lo - 8 + col;
-            wv[idx] = (y >= 0 && y < d0.bh && x >= 0 && x < d0.bw) ? words[d0.base + y * d0.bw + x] : 0u;
-          }
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // Index similarity: each offset's score gains sim[number of pixels
+      // whose index differs between the own word and the candidate's].  The
+      // tile's rows are taken in segments that lie in one row of one frame
+      // (normally the whole tile): the segment's window words, 16 window rows
+      // x 47 columns, are staged in the wave's LDS (coalesced loads, all
+      // requested at once) and its lanes add the scores branch-free.  An
+      // offset that is not admitted reads a staged 0 or a neighbour's word
+      // and gains a score too, but its key is masked out of the ranking.
+      const uint32_t ow = act ? (uint32_t)(own[row] >> 32) : 0u;
+      uint32_t* wv = s_win[threadIdx.x >> 6];
+      const int g_end = min(rows, tile * 32 + 32);
+      for (int g_lo = tile * 32; g_lo < g_end;) {  // wave-uniform
+        const FrameDev& d0 = fr[sid[g_lo]];
+        const int k0 = g_lo - d0.base, by0 = k0 / d0.bw, bx_lo = k0 % d0.bw;
+        const int seg_end = g_lo + min(g_end - g_lo, d0.bw - bx_lo);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the last segment's reads are done
+        __builtin_amdgcn_wave_barrier();
+        constexpr int kStage = (kNetWin * kWinCols + 63) / 64;  // 12 words per lane
+        uint32_t sw[kStage];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) {
+          const int idx = lane + 64 * q;
+          const int r = idx / kWinCols, col = idx - r * kWinCols;
+          const int y = net == 0 ? by0 + r - 8 : by0 - r, x = bx_lo - 8 + col;
+          sw[q] = (idx < kNetWin * kWinCols && y >= 0 && y < d0.bh && x >= 0 && x < d0.bw)
+                      ? words[d0.base + y * d0.bw + x]
+                      : 0u;
         }
-      }
-      if (act) {
-        const uint32_t ow = (uint32_t)(own[row] >> 32);
-        const uint32_t* wv = s_win[threadIdx.x >> 6] + (fbx - bx_lo);
 #pragma unroll
-        for (int nb = 0; nb < 8; ++nb)
+        for (int q = 0; q < kStage; ++q)
+          if (lane + 64 * q < kNetWin * kWinCols) wv[lane + 64 * q] = sw[q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (act && row >= g_lo && row < seg_end) {
+          const uint32_t* wl = wv + (fbx - bx_lo);
 #pragma unroll
-          for (int i = 0; i < 16; ++i) {
-            const int bit = 8 * (i >> 2) + 4 * h + (i & 3), L = nb * 32 + bit;
-            if ((vw[nb] >> bit) & 1u) {
-              const uint32_t w = staged ? wv[(L >> 4) * kWinCols + (L & 15)]
-                                        : words[offset_block(*dp, net, fbx, fby, L)];
-              acc[nb][i] += s_sim[index_diff(ow, w)];
+          for (int nb = 0; nb < 8; ++nb) {
+            uint32_t wd[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              const int L = nb * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+              wd[i] = wl[(L >> 4) * kWinCols + (L & 15)];
             }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[nb][i] += s_sim[index_diff(ow, wd[i])];
           }
+        }
+        g_lo = seg_end;
       }
     }
     uint32_t top[K];
