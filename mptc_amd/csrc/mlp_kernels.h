@@ -713,37 +713,38 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
                                                      long tf_stride) {
   __shared__ uint8_t s_snap[512];
   __shared__ float2 s_ab[4];
-  __shared__ unsigned long long s_acc;
+  __shared__ unsigned long long s_cnt[2][4];  // per wave: first evaluations, re-evaluations
   __shared__ unsigned int s_chg, s_prev;
-  // After a pass that changed no word every evaluation is current: nothing
-  // to do.  The previous pass's count is read once, before this workgroup
-  // reports its end (the last workgroup to end clears it).  It is the same
-  // for every workgroup (only the last one to report changes it), so either
-  // all return here or none does; an empty pass reports nothing: its counter
-  // stays zero and the previous pass already wrote 0 to *conv.
-  if (threadIdx.x == 0) s_prev = first ? 1u : st->n_changed[par ^ 1];
-  __syncthreads();
-  if (s_prev == 0) {
-    report_fixpoint(conv);
-    return;
-  }
-  load_lds_tables(s_snap, s_ab);
-  if (threadIdx.x == 0) s_chg = 0;
-  __syncthreads();
   const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int lane = (int)(threadIdx.x & 63);
-  unsigned long long n_cand = 0, n_re = 0;  // first evaluations, re-evaluations
+  // The previous pass's count, the evaluation tables and this wave's tile
+  // flag are requested together, before the one barrier.  After a pass that
+  // changed no word every evaluation is current: nothing to do.  The
+  // previous pass's count is read once, before this workgroup reports its end
+  // (the last workgroup to end clears it).  It is the same for every
+  // workgroup (only the last one to report changes it), so either all return
+  // here or none does; an empty pass reports nothing: its counter stays zero
+  // and the previous pass already wrote 0 to *conv.
   // Tiles to do: every tile in the first pass; later, the tiles the previous
   // pass flagged (a word some block of the tile may propose changed, or the
   // tile did not settle).  The other tiles' evaluations are all current.  A
   // pass reads and clears the flags of parity par ^ 1 and sets those of
   // parity par; stale set flags only cost a redo.
-  bool todo = false;
-  if (wv < n_itiles) {
-    uint8_t* fl = tflag + (size_t)(par ^ 1) * tf_stride + wv;
-    todo = first || *fl != 0;
-    if (lane == 0) *fl = 0;
+  if (threadIdx.x == 0) {
+    s_prev = first ? 1u : st->n_changed[par ^ 1];
+    s_chg = 0;
   }
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) s_snap[i] = c_snap[i];
+  init_ab_table(s_ab);
+  uint8_t* const fl = wv < n_itiles ? tflag + (size_t)(par ^ 1) * tf_stride + wv : nullptr;
+  const bool todo = fl && (first || *fl != 0);
+  __syncthreads();
+  if (s_prev == 0) {
+    report_fixpoint(conv);
+    return;
+  }
+  if (fl && lane == 0) *fl = 0;
+  unsigned long long n_cand = 0, n_re = 0;  // first evaluations, re-evaluations
   if (todo) {
     int f = 0;
     while (f + 1 < n_frames && fr[f + 1].itile_base <= wv) ++f;
@@ -931,13 +932,29 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
       }
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0 && s_chg) {
-    const unsigned old = atomicAdd(&st->n_changed[par], s_chg);
-    if (old == 0) atomicAdd(&st->active_passes, 1u);
+  // the workgroup's counts through one barrier: wave sums, then thread 0
+  for (int o = 32; o > 0; o >>= 1) {
+    n_cand += __shfl_xor(n_cand, o);
+    n_re += __shfl_xor(n_re, o);
   }
-  wg_count(&s_acc, n_cand, st->intra_cands);
-  wg_count(&s_acc, n_re, st->reeval_cands);
+  if (lane == 0) {
+    s_cnt[0][threadIdx.x >> 6] = n_cand;
+    s_cnt[1][threadIdx.x >> 6] = n_re;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_chg) {
+      const unsigned old = atomicAdd(&st->n_changed[par], s_chg);
+      if (old == 0) atomicAdd(&st->active_passes, 1u);
+    }
+    unsigned long long c0 = 0, c1 = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      c0 += s_cnt[0][w];
+      c1 += s_cnt[1][w];
+    }
+    if (c0) atomicAdd(&st->intra_cands[blockIdx.x % kStatSlots], c0);
+    if (c1) atomicAdd(&st->reeval_cands[blockIdx.x % kStatSlots], c1);
+  }
   pass_end(st, par, &st->n_changed[par ^ 1], conv);
 }
 
