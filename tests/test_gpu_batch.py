@@ -192,3 +192,42 @@ def test_calls_on_alternating_streams(orc):
     sa.destroy()
     sb.destroy()
     enc.close()
+
+
+def test_learned_batch_matches_single_streams(orc):
+    """Learned mode with index-similarity scores on both networks: a batch of
+    uneven streams gives each stream the same bytes as encoding it alone (a
+    block's proposals and intra decisions depend only on its own frame).  In
+    the batch the fused top-k's tiles of 32 blocks span two rows of the
+    200x120 frame (50 blocks per row) and the boundaries between frames."""
+    from mptc_amd import hip, lib
+    from oracle import SynthCfg
+    sizes = [(200, 120, 21), (96, 64, 22), (256, 128, 23)]
+    frames = [orc.synth_frames(SynthCfg(w, h, sd, 6, 1, 4, 8, 100), 3) for w, h, sd in sizes]
+    S, thr, k = 8, 2, 4
+    rng = np.random.default_rng(5)
+    sims = [rng.normal(0, 3, lib.SIM_BINS).astype(np.float32) for _ in (0, 1)]
+
+    def learned(enc):
+        for net in (0, 1):
+            enc.net_init(net, 61 + net)
+            enc.net_set_similarity(net, sims[net])
+        enc.set_search_mode(lib.SEARCH_LEARNED, k)
+
+    total = sum((w // 4) * (h // 4) for w, h, _ in sizes)
+    enc = lib.Encoder(256, 128, S, thr, max_batch_frames=3, max_batch_blocks=total)
+    learned(enc)
+    log(f"learned batch: {len(sizes)} streams, {total} blocks per step")
+    res = _encode_streams(enc, [Stream(hip, fr) for fr in frames], 3, 2)
+    enc.close()
+    reused = 0
+    for i, fr in enumerate(frames):
+        one_enc = lib.Encoder(fr.shape[2], fr.shape[1], S, thr)
+        learned(one_enc)
+        one = _encode_streams(one_enc, [Stream(hip, fr)], 3, 2)
+        one_enc.close()
+        for f in range(3):
+            compare_results(res[f][i], one[f][0], f"stream {i} frame {f}")
+            assert np.array_equal(res[f][i]["kind"], one[f][0]["kind"])
+            reused += int((res[f][i]["kind"] != 0).sum())
+    assert reused > 0

@@ -513,14 +513,18 @@ def _cand_block(net, bx, by, bw, L):
     return y * bw + bx + dx
 
 
+@pytest.mark.parametrize("size", [(256, 256), (200, 120)])
 @pytest.mark.parametrize("source", ["network", "prior"])
-def test_similarity_scores_enter_the_ranking(env, source):
+def test_similarity_scores_enter_the_ranking(env, source, size):
     """With index-similarity scores set, the proposals are the top-k admitted
     offsets of (logit or prior score) + sim[h], h = the pixels whose index
     differs between the block's own word and the candidate's word (inter: the
-    previous frame's word; intra: the word the intra search starts from)."""
+    previous frame's word; intra: the word the intra search starts from).
+    At 200x120 (50 blocks per row) the fused top-k's tiles of 32 blocks span
+    two rows of the frame."""
     hip, lib = env
     spec = dict(SEQUENCES["c1_s8_t8"])
+    spec["width"], spec["height"] = size
     frames = lib.synth_frames(synth_cfg(spec), 0, 2)
     seq = Seq(hip, frames)
     S, k = 8, 8
