@@ -183,3 +183,14 @@ class Event:
         check(rt().hipEventElapsedTime(C.byref(ms), C.c_void_p(self.handle), C.c_void_p(end.handle)),
               "hipEventElapsedTime")
         return float(ms.value)
+
+    def destroy(self):
+        if self.handle:
+            rt().hipEventDestroy(C.c_void_p(self.handle))
+            self.handle = None
+
+
+def stream_wait(stream: Stream, event: Event):
+    """Work queued on `stream` after this call waits for `event` (hipStreamWaitEvent)."""
+    check(rt().hipStreamWaitEvent(C.c_void_p(stream.handle), C.c_void_p(event.handle), C.c_uint(0)),
+          "hipStreamWaitEvent")
