@@ -695,10 +695,12 @@ __global__ void __launch_bounds__(256) k_block_prep(const FrameDev* __restrict__
 // only earlier blocks, so a pass that changes no word ends at the unique
 // fixpoint.  A tile not settled after kMaxRounds rounds stores, for every
 // lane, the decision its current evaluations give; what is stale is redone in
-// the next pass (never-done evaluations are stored as kNotEvaluated).  Pass
-// counters: this pass adds its changed words to n_changed[par] (zero at its
-// start) and reads the previous pass's, n_changed[par ^ 1], which its last
-// workgroup clears for the next pass (pass_end); the count also goes to *conv.
+// the next pass (never-done evaluations are stored as kNotEvaluated), and the
+// tile counts as one change, so that a next pass follows even when no word
+// changed.  Pass counters: this pass adds its changed words (and unsettled
+// tiles) to n_changed[par] (zero at its start) and reads the previous pass's,
+// n_changed[par ^ 1], which its last workgroup clears for the next pass
+// (pass_end); the count also goes to *conv.
 constexpr int kMaxRounds = 512;
 constexpr int kNotEvaluated = (int)0x80000000;
 
@@ -910,7 +912,10 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
       }
     }
     if (act && !clean_once) decide();  // round limit reached: store a consistent state
-    if (!settled && lane == 0) tflag[(size_t)par * tf_stride + wv] = 1;  // redo what is stale next pass
+    if (!settled && lane == 0) {  // redo what is stale next pass
+      tflag[(size_t)par * tf_stride + wv] = 1;
+      atomicAdd(&s_chg, 1u);
+    }
     if (published) {
 #pragma unroll
       for (int i = 0; i < K; ++i)
