@@ -174,7 +174,8 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
  * "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine",
  * "intra_apply", "intra_reeval", "intra_tile", "commit", "unique";
  * learned search: "net_input", "net_l12", "net_l3" (inference), "propose_inter",
- * "top_intra", "lintra_pass"; "net_forward": training products; pruned search:
+ * "top_intra", "lintra_pass"; "net_forward": training steps (the fused
+ * forward/loss/backward, weight-gradient and reduction/Adam launches); pruned search:
  * "pruned_inter", "pruned_intra", "fold_intra") since it was enabled
  * (synchronises).  While enabled, steps run as individual launches, not graphs. */
 int32_t mptc_profile_enable(mptc_ctx* ctx, int32_t enable);
