@@ -97,8 +97,11 @@ def test_set_words_layout():
     assert np.array_equal((q > 0).sum(axis=1), [1, 1, 1])
 
 
-@pytest.mark.parametrize("n,seed_w,density", [(64, 1.0, 0.0), (513, 0.25, 0.05), (3000, 4.0, 0.3)])
+@pytest.mark.parametrize("n,seed_w,density", [(1, 1.0, 0.2), (47, 2.0, 0.1), (64, 1.0, 0.0), (513, 0.25, 0.05),
+                                             (3000, 4.0, 0.3)])
 def test_gradients_match_oracle(env, n, seed_w, density):
+    """Gradients of the training step (batch rows padded to 32: 1 and 47 rows
+    leave whole and partial workgroups of padding) against the oracle."""
     hip, lib, enc = env
     log(f"gradients n={n}")
     rng = np.random.default_rng(100 + n)
