@@ -176,7 +176,9 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
  * learned search: "net_input", "net_l12", "net_l3" (inference), "propose_inter",
  * "top_intra", "lintra_pass"; "net_forward": training steps (the fused
  * forward/loss/backward, weight-gradient and reduction/Adam launches); pruned search:
- * "pruned_inter", "pruned_intra", "fold_intra") since it was enabled
+ * "pruned_inter", "pruned_intra", "fold_intra" (only when the window is split
+ * into a number of chunks other than 1, 2 or 4; otherwise the intra search
+ * takes those decisions)) since it was enabled
  * (synchronises).  While enabled, steps run as individual launches, not graphs. */
 int32_t mptc_profile_enable(mptc_ctx* ctx, int32_t enable);
 int32_t mptc_profile_read(mptc_ctx* ctx, const char* name, int64_t* launches, double* total_ms);

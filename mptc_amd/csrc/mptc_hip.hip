@@ -602,7 +602,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
     if ((rc = launch(c, s, KF_PRUNED_INTER, [&] {
            hipLaunchKernelGGL(k_pruned_search<false>, dim3(gwk256), dim3(256), 0, s, c->d_fr, c->d_tsid, c->n_wtiles,
                               tw, K, S, thr, c->d_own, c->d_need, c->d_rw, c->d_rtab, c->d_rbt, c->d_orig, nullptr,
-                              c->nbs, nullptr, c->d_summ, c->d_stats);
+                              c->nbs, nullptr, c->d_summ, c->d_stats, nullptr, nullptr, nullptr, 0);
          })))
       return rc;
     if ((rc = launch(c, s, KF_INTER_COMBINE, [&] {
@@ -647,13 +647,16 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
     return lintra_pass_q(c, s, 0, 1, false);
   }
   if (prune) {  // bound tables and cleared passing sets from k_intra_prep
+    // with 1, 2 or 4 chunks every tile's chunks are waves of one workgroup,
+    // and the search also takes the intra decisions (k_fold_intra's work)
+    const int fold = 4 % K == 0 ? 1 : 0;
     if ((rc = launch(c, s, KF_PRUNED_INTRA, [&] {
            hipLaunchKernelGGL(k_pruned_search<true>, dim3(gwk256), dim3(256), 0, s, c->d_fr, c->d_tsid, c->n_wtiles,
                               tw, K, S, thr, c->d_own, c->d_need, c->d_cw, c->d_ctab, c->d_cbt, c->d_orig, c->d_E,
-                              c->nbs, c->d_pm, nullptr, c->d_stats);
+                              c->nbs, c->d_pm, nullptr, c->d_stats, c->d_intra, c->d_cwn, c->d_chg, fold);
          })))
       return rc;
-    if ((rc = launch(c, s, KF_FOLD_INTRA, [&] {
+    if (!fold && (rc = launch(c, s, KF_FOLD_INTRA, [&] {
            hipLaunchKernelGGL(k_fold_intra, dim3(grid_for(total, 256)), dim3(256), 0, s, c->d_fr, c->d_sid,
                               total, S, thr, c->d_own, c->d_need, c->d_cw, c->d_E, c->nbs, c->d_pm, c->d_intra,
                               c->d_cwn, c->d_chg, c->d_stats);

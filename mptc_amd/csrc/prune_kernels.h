@@ -25,7 +25,9 @@
 //                   k_inter_combine); intra: E for the fixpoint passes
 //   k_fold_intra    per block, the reference's selection over the passing
 //                   deltas of E: the intra decision with the changed-word list
-//                   (as k_intra_combine)
+//                   (as k_intra_combine).  With 1, 2 or 4 chunks a workgroup
+//                   holds every chunk of its tiles and the intra search takes
+//                   these decisions itself, after a workgroup barrier.
 //
 // The words' bound tables (BoundTab, batch_kernels.h) are built by
 // k_block_prep (the previous frame's words) and k_intra_prep (the current
@@ -170,7 +172,9 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
                                                        const float4* __restrict__ tabs,
                                                        const BoundTab* __restrict__ btab, const int* __restrict__ orig,
                                                        uint2* __restrict__ E4, size_t nbs, uint32_t* __restrict__ pm,
-                                                       uint4* __restrict__ summ, DevStats* __restrict__ st) {
+                                                       uint4* __restrict__ summ, DevStats* __restrict__ st,
+                                                       uint64_t* __restrict__ dec, uint32_t* __restrict__ cw_new,
+                                                       int* __restrict__ chg_list, int fold) {
   __shared__ uint8_t s_snap[512];
   __shared__ float2 s_ab[4];
   __shared__ uint32_t s_blk[4][kPackedBlk][64];      // [wave][field][lane]
@@ -179,6 +183,7 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
   __shared__ uint8_t s_pass[4][64];                      // inter: the lane's batch holds a passing delta
   __shared__ int16_t s_cf[4][kFoldWords][64];            // inter: each lane's ChunkFold state and row
   __shared__ unsigned long long s_acc;
+  __shared__ unsigned int s_base;
   load_lds_tables(s_snap, s_ab);
   int f = 0, bx = 0, by = 0, chunk = 0;
   const bool in = tile_block(fr, tsid, n_wtiles, tw, n_chunks, f, bx, by, chunk);
@@ -351,6 +356,26 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
     cf_load(s_cf[wave], lane, cf, cf_row);
     cf.end_row(cf_row);
     summ[(size_t)chunk * nbs + g] = cf.pack();
+  }
+  if (kIntra && fold) {
+    // Intra decisions (as k_fold_intra), when every tile's chunks are waves of
+    // this workgroup (n_chunks divides 4): once all of them have written
+    // their passing sets and E, the chunk-0 wave of each tile folds its
+    // blocks.  The workgroup's waves share the CU's L1, so the barrier
+    // orders those writes before the reads.
+    __syncthreads();
+    bool changed = false;
+    if (chunk == 0 && act) {
+      const Fold fd = fold_passing_set(pm, E4, nbs, g, S);
+      const uint64_t dc = intra_decision(words, fbase, fbw, bx, by, S, thr, fd);
+      dec[g] = dc;
+      const uint32_t w = dec_acc(dc) ? dec_word(dc) : (uint32_t)(own[g] >> 32);
+      if (w != words[g]) {
+        cw_new[g] = w;
+        changed = true;
+      }
+    }
+    wg_append(changed, g, chg_list, &st->n_changed[0], &s_base);
   }
   wg_count(&s_acc, n_bound, kIntra ? st->intra_cands : st->inter_cands);
   wg_count(&s_acc, n_exact, kIntra ? st->exact_intra : st->exact_inter);
