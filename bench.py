@@ -162,7 +162,7 @@ def cpu_baseline(spec, steps):
     return one, allc, host
 
 
-KERNELS = ("fit", "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine", "intra_apply",
+KERNELS = ("fit", "inter", "intra_prep", "intra_dense", "intra_combine", "intra_apply",
            "intra_reeval", "intra_tile", "commit", "unique", "net_input", "net_l12", "net_l3",
            "propose_inter", "top_intra", "lintra_pass", "pruned_inter", "pruned_intra", "fold_intra")
 

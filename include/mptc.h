@@ -171,7 +171,8 @@ int32_t mptc_get_stats(mptc_ctx* ctx, mptc_frame_stats* out);
  * mptc_profile_enable(ctx, 1) starts recording; mptc_profile_read returns the
  * accumulated launch count and milliseconds of kernel `name` ("fit": the step's
  * first kernel: counters cleared, fit, reference tables, network input),
- * "inter", "inter_combine", "intra_prep", "intra_dense", "intra_combine",
+ * "inter", "intra_prep" (with the inter decisions of an exhaustive inter
+ * search), "intra_dense", "intra_combine",
  * "intra_apply", "intra_reeval", "intra_tile", "commit", "unique";
  * learned search: "net_input", "net_l12", "net_l3" (inference), "propose_inter",
  * "top_intra", "lintra_pass"; "net_forward": training steps (the fused

@@ -300,7 +300,8 @@ __global__ void __launch_bounds__(256) k_fit(const FrameDev* __restrict__ fr, co
 // InterBlockSearch (codec/dxt_image.cpp:715-774): window rows j = by-S+r
 // (r ascending), columns i = bx-S+t (t ascending), position r*2S + t.  Each
 // work item scans `rows` rows of the window and leaves a ChunkFold summary;
-// k_inter_combine folds them and applies the accept test (:890).
+// inter_decision (in k_intra_prep) folds them and applies the accept test
+// (:890).
 //
 // With amask non-null, also records which offsets pass the accept test:
 // amask[g * 16 + dy + 8] bit dx + 8 for the offsets (dx, dy) of the 16x16
@@ -352,15 +353,11 @@ __global__ void __launch_bounds__(256) k_inter(const FrameDev* __restrict__ fr, 
   wg_count(&s_acc, n_cand, st->inter_cands);
 }
 
-__global__ void __launch_bounds__(256) k_inter_combine(const FrameDev* __restrict__ fr,
-                                                       const uint8_t* __restrict__ sid, int total, int n_chunks,
-                                                       int S, int thr, const uint32_t* __restrict__ rw,
-                                                       const uint4* __restrict__ summ, size_t sstride,
-                                                       uint64_t* __restrict__ dec_out) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const FrameDev& d = fr[sid[g]];
-  if (d.is_intra) return;
+// The inter decision of block g of an inter frame from its chunk summaries
+// (selected position and returned delta) and the previous frame's words rw.
+__device__ __forceinline__ uint64_t inter_decision(const FrameDev& d, int g, int n_chunks, int S, int thr,
+                                                   const uint32_t* __restrict__ rw, const uint4* __restrict__ summ,
+                                                   size_t sstride) {
   int sel, ret;
   combine_chunks(summ, sstride, g, n_chunks, sel, ret);
   uint64_t dc = 0;
@@ -370,19 +367,23 @@ __global__ void __launch_bounds__(256) k_inter_combine(const FrameDev* __restric
     const uint32_t c = rw[d.base + (by - S + r) * d.bw + (bx - S + t)];
     dc = pack_dec(c, (uint32_t)t | 0x80u, (uint32_t)r | 0x80u, true);
   }
-  dec_out[g] = dc;
+  return dc;
 }
 
 // Blocks that need the intra search, the current words the search starts
 // from (inter-reused words are final: they read only the previous frame;
 // every other block starts from its own word), and for the exact intra
 // search (orig not null) each block's own error, the current words' tables
-// and cleared pass flags.
+// and cleared pass flags.  With summ (not null: after an exhaustive inter
+// search) the blocks of inter frames first take their inter decisions from
+// the chunk summaries and write them to inter_dec; otherwise inter_dec holds
+// them already (learned proposals).
 __global__ void __launch_bounds__(256) k_intra_prep(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid, int total,
-                             const uint64_t* __restrict__ own, const uint64_t* __restrict__ inter_dec,
+                             const uint64_t* __restrict__ own, uint64_t* __restrict__ inter_dec,
                              uint8_t* __restrict__ need, uint32_t* __restrict__ cw, float4* __restrict__ ctab,
                              uint32_t* __restrict__ dirty, int* __restrict__ orig, BoundTab* __restrict__ cbt,
-                             uint4* __restrict__ pm4) {
+                             uint4* __restrict__ pm4, int n_chunks, int S, int thr,
+                             const uint32_t* __restrict__ rw, const uint4* __restrict__ summ, size_t sstride) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const FrameDev& d = fr[sid[g]];
@@ -391,7 +392,13 @@ __global__ void __launch_bounds__(256) k_intra_prep(const FrameDev* __restrict__
   const uint64_t o = own[g];
   uint32_t w = (uint32_t)(o >> 32);
   if (!d.is_intra) {
-    const uint64_t dc = inter_dec[g];
+    uint64_t dc;
+    if (summ) {
+      dc = inter_decision(d, g, n_chunks, S, thr, rw, summ, sstride);
+      inter_dec[g] = dc;
+    } else {
+      dc = inter_dec[g];
+    }
     if (dec_acc(dc)) {
       inter_ok = true;
       w = dec_word(dc);

@@ -104,12 +104,12 @@ int init_tables(int device) {
 }
 
 enum KernelId {
-  KF_FIT, KF_INTER, KF_INTER_COMBINE, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
+  KF_FIT, KF_INTER, KF_PREP, KF_DENSE, KF_INTRA_COMBINE, KF_APPLY, KF_REEVAL,
   KF_TILE, KF_COMMIT, KF_UNIQUE, KF_NET_INPUT, KF_NET_FWD, KF_PROPOSE, KF_TOP_INTRA, KF_LPASS, KF_NET_L12,
   KF_NET_L3, KF_PRUNED_INTER, KF_PRUNED_INTRA, KF_FOLD_INTRA, KF_N
 };
 const char* kKernelNames[KF_N] = {"fit",          "inter",
-                                  "inter_combine", "intra_prep",  "intra_dense", "intra_combine",
+                                  "intra_prep",   "intra_dense", "intra_combine",
                                   "intra_apply",  "intra_reeval", "intra_tile", "commit",
                                   "unique",       "net_input",   "net_forward", "propose_inter",
                                   "top_intra",    "lintra_pass",  "net_l12",     "net_l3",
@@ -592,6 +592,9 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
        })))
     return rc;
   const int K = chunks_for(S, c->tune.win_chunks);
+  // the exhaustive inter searches leave chunk summaries, from which
+  // k_intra_prep takes the inter decisions
+  const uint4* inter_summ = nullptr;
   const unsigned gwk = grid_for(c->n_wtiles * K * 64, wg);
   const unsigned gwk256 = grid_for(c->n_wtiles * K * 64, 256);
   if (!c->any_inter) {
@@ -605,11 +608,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
                               c->nbs, nullptr, c->d_summ, c->d_stats, nullptr, nullptr, nullptr, 0);
          })))
       return rc;
-    if ((rc = launch(c, s, KF_INTER_COMBINE, [&] {
-           hipLaunchKernelGGL(k_inter_combine, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
-                              K, S, thr, c->d_rw, c->d_summ, c->nbs, c->d_inter);
-         })))
-      return rc;
+    inter_summ = c->d_summ;  // k_intra_prep takes the inter decisions
   } else if (c->mode == MPTC_SEARCH_EXHAUSTIVE) {
     if ((rc = launch(c, s, KF_INTER, [&] {
            hipLaunchKernelGGL(k_inter, dim3(gwk), dim3(wg), 0, s, c->d_fr, c->d_tsid, c->n_wtiles, tw, K, S,
@@ -617,11 +616,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
                               c->record ? c->d_amask[0] : nullptr);
          })))
       return rc;
-    if ((rc = launch(c, s, KF_INTER_COMBINE, [&] {
-           hipLaunchKernelGGL(k_inter_combine, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
-                              K, S, thr, c->d_rw, c->d_summ, c->nbs, c->d_inter);
-         })))
-      return rc;
+    inter_summ = c->d_summ;  // k_intra_prep takes the inter decisions
   } else {
     if ((rc = proposals_q(c, s, MPTC_NET_INTER, nullptr, c->d_pinter, x_done))) return rc;
     // the learned intra search needs only k_intra_prep's flags and starting
@@ -639,7 +634,8 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
          hipLaunchKernelGGL(k_intra_prep, dim3(grid_for(total, wg)), dim3(wg), 0, s, c->d_fr, c->d_sid, total,
                             c->d_own, c->d_inter, c->d_need, c->d_cw, lrn ? nullptr : c->d_ctab,
                             lrn ? nullptr : c->d_dirty, lrn ? nullptr : c->d_orig, prune ? c->d_cbt : nullptr,
-                            prune ? reinterpret_cast<uint4*>(c->d_pm) : nullptr);
+                            prune ? reinterpret_cast<uint4*>(c->d_pm) : nullptr, K, S, thr, c->d_rw, inter_summ,
+                            c->nbs);
        })))
     return rc;
   if (c->mode == MPTC_SEARCH_LEARNED) {

@@ -22,7 +22,8 @@
 //                   non-passing -> kPruned; the wave's survivors are compacted
 //                   in LDS and evaluated exactly, 64 per round.  Inter: the
 //                   chunk summaries of the selection rule (as k_inter, then
-//                   k_inter_combine); intra: E for the fixpoint passes
+//                   inter_decision in k_intra_prep); intra: E for the
+//                   fixpoint passes
 //   k_fold_intra    per block, the reference's selection over the passing
 //                   deltas of E: the intra decision with the changed-word list
 //                   (as k_intra_combine).  With 1, 2 or 4 chunks a workgroup
