@@ -61,6 +61,13 @@ struct FrameDev {
 // address; the host sums the slots.
 constexpr int kStatSlots = 32;
 
+// pass_end counts finished workgroups in two levels: groups of kPassGroup
+// workgroups count into their own counter (one 256-byte line each), and the
+// last workgroup of each group into pass_done.
+constexpr int kPassGroup = 64;
+constexpr int kPassGroups = 64;  // grids of up to 4,096 workgroups; larger ones count flat
+constexpr int kPassStride = 64;  // words between the group counters
+
 struct DevStats {
   unsigned long long inter_cands[kStatSlots], intra_cands[kStatSlots], reeval_cands[kStatSlots];
   unsigned long long exact_inter[kStatSlots], exact_intra[kStatSlots];  // pruned search: exact evaluations
@@ -69,6 +76,7 @@ struct DevStats {
   unsigned int n_changed[2];             // length of the changed-word list, by pass parity
   unsigned int pass_done;                // workgroups of the running fixpoint pass that finished
   unsigned int skip_tile;                // the last pass was tile-resolved (k_intra_tile_pass)
+  unsigned int pass_grp[kPassGroups * kPassStride];  // pass_end's group counters
 };
 
 // The pruning bound's table of an index word (prune_kernels.h).
@@ -132,12 +140,24 @@ __device__ __forceinline__ void wg_count(unsigned long long* s_acc, unsigned lon
 // rearms pass_done and writes the pass's count n_changed[par] to `conv`
 // (pinned host memory the host polls for convergence; null: not written).
 // The workgroups' counter updates are atomics whose results they consume
-// before they get here, so they are complete before their pass_done add (no
-// device-scope fence, which would write back the XCD's L2).
+// before they get here, so they are complete before their count (no
+// device-scope fence, which would write back the XCD's L2).  The count is
+// two-level (kPassGroup workgroups per group counter, then the groups), so
+// that no one address takes every workgroup's atomic; a group's last
+// workgroup has its group counter's result before it counts the group.
 __device__ __forceinline__ void pass_end(DevStats* st, int par, unsigned int* clear, unsigned int* conv) {
   if (threadIdx.x != 0) return;
-  const unsigned t = atomicAdd(&st->pass_done, 1u);
-  if (t + 1 != gridDim.x) return;
+  const unsigned n_wg = gridDim.x, ng = (n_wg + kPassGroup - 1) / kPassGroup;
+  if (ng > 1 && ng <= (unsigned)kPassGroups) {
+    const unsigned grp = blockIdx.x / kPassGroup;
+    const unsigned members = min((unsigned)kPassGroup, n_wg - grp * kPassGroup);
+    unsigned int* gc = &st->pass_grp[grp * kPassStride];
+    if (atomicAdd(gc, 1u) + 1 != members) return;
+    atomicExch(gc, 0u);  // rearmed for the next pass
+    if (atomicAdd(&st->pass_done, 1u) + 1 != ng) return;
+  } else if (atomicAdd(&st->pass_done, 1u) + 1 != n_wg) {
+    return;
+  }
   const unsigned n = atomicAdd(&st->n_changed[par], 0u);
   if (clear) atomicExch(clear, 0u);
   atomicExch(&st->pass_done, 0u);
