@@ -57,9 +57,13 @@ struct FrameDev {
 };
 
 // Statistics counters are spread over kStatSlots copies (slot = workgroup
-// index mod kStatSlots) so that thousands of workgroups do not queue on one
-// address; the host sums the slots.
+// index mod kStatSlots), each on a 128-byte line of its own, so that
+// thousands of workgroups do not queue on one address or line; the host sums
+// the slots.  Slot i of a 64-bit counter array is element i * kSlot64, of a
+// 32-bit one element i * kSlot32.
 constexpr int kStatSlots = 32;
+constexpr int kSlot64 = 16;
+constexpr int kSlot32 = 32;
 
 // pass_end counts finished workgroups in two levels: groups of kPassGroup
 // workgroups count into their own counter (one 256-byte line each), and the
@@ -69,9 +73,10 @@ constexpr int kPassGroups = 64;  // grids of up to 4,096 workgroups; larger ones
 constexpr int kPassStride = 64;  // words between the group counters
 
 struct DevStats {
-  unsigned long long inter_cands[kStatSlots], intra_cands[kStatSlots], reeval_cands[kStatSlots];
-  unsigned long long exact_inter[kStatSlots], exact_intra[kStatSlots];  // pruned search: exact evaluations
-  unsigned int n_inter[kStatSlots], n_intra[kStatSlots];
+  unsigned long long inter_cands[kStatSlots * kSlot64], intra_cands[kStatSlots * kSlot64];
+  unsigned long long reeval_cands[kStatSlots * kSlot64];
+  unsigned long long exact_inter[kStatSlots * kSlot64], exact_intra[kStatSlots * kSlot64];  // pruned search: exact evaluations
+  unsigned int n_inter[kStatSlots * kSlot32], n_intra[kStatSlots * kSlot32];
   unsigned int n_unique, active_passes;  // passes that changed a word
   unsigned int n_changed[2];             // length of the changed-word list, by pass parity
   unsigned int pass_done;                // workgroups of the running fixpoint pass that finished
@@ -130,7 +135,7 @@ __device__ __forceinline__ void wg_count(unsigned long long* s_acc, unsigned lon
   __syncthreads();
   if (v) atomicAdd(s_acc, v);
   __syncthreads();
-  if (threadIdx.x == 0 && *s_acc) atomicAdd(&slots[blockIdx.x % kStatSlots], *s_acc);
+  if (threadIdx.x == 0 && *s_acc) atomicAdd(&slots[(blockIdx.x % kStatSlots) * kSlot64], *s_acc);
 }
 
 // End of a fixpoint pass's workgroup (thread 0 acts; call it from every
@@ -1095,8 +1100,8 @@ __global__ void __launch_bounds__(256) k_commit(const FrameDev* __restrict__ fr,
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (s_cnt[0]) atomicAdd(&st->n_intra[blockIdx.x % kStatSlots], s_cnt[0]);
-    if (s_cnt[1]) atomicAdd(&st->n_inter[blockIdx.x % kStatSlots], s_cnt[1]);
+    if (s_cnt[0]) atomicAdd(&st->n_intra[(blockIdx.x % kStatSlots) * kSlot32], s_cnt[0]);
+    if (s_cnt[1]) atomicAdd(&st->n_inter[(blockIdx.x % kStatSlots) * kSlot32], s_cnt[1]);
   }
 }
 

@@ -957,8 +957,8 @@ __global__ void __launch_bounds__(256) k_lintra_pass(const FrameDev* __restrict_
       c0 += s_cnt[0][w];
       c1 += s_cnt[1][w];
     }
-    if (c0) atomicAdd(&st->intra_cands[blockIdx.x % kStatSlots], c0);
-    if (c1) atomicAdd(&st->reeval_cands[blockIdx.x % kStatSlots], c1);
+    if (c0) atomicAdd(&st->intra_cands[(blockIdx.x % kStatSlots) * kSlot64], c0);
+    if (c1) atomicAdd(&st->reeval_cands[(blockIdx.x % kStatSlots) * kSlot64], c1);
   }
   pass_end(st, par, &st->n_changed[par ^ 1], conv);
 }

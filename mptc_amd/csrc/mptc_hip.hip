@@ -1126,13 +1126,13 @@ int32_t mptc_get_stats(mptc_ctx* c, mptc_frame_stats* out) {
   out->n_inter = out->n_intra = out->inter_candidates = out->intra_candidates = out->reeval_candidates = 0;
   out->exact_inter_candidates = out->exact_intra_candidates = 0;
   for (int i = 0; i < kStatSlots; ++i) {
-    out->n_inter += d.n_inter[i];
-    out->n_intra += d.n_intra[i];
-    out->inter_candidates += (int64_t)d.inter_cands[i];
-    out->intra_candidates += (int64_t)d.intra_cands[i];
-    out->reeval_candidates += (int64_t)d.reeval_cands[i];
-    out->exact_inter_candidates += (int64_t)d.exact_inter[i];
-    out->exact_intra_candidates += (int64_t)d.exact_intra[i];
+    out->n_inter += d.n_inter[i * kSlot32];
+    out->n_intra += d.n_intra[i * kSlot32];
+    out->inter_candidates += (int64_t)d.inter_cands[i * kSlot64];
+    out->intra_candidates += (int64_t)d.intra_cands[i * kSlot64];
+    out->reeval_candidates += (int64_t)d.reeval_cands[i * kSlot64];
+    out->exact_inter_candidates += (int64_t)d.exact_inter[i * kSlot64];
+    out->exact_intra_candidates += (int64_t)d.exact_intra[i * kSlot64];
   }
   out->n_unique = d.n_unique;
   out->intra_iterations = (int32_t)d.active_passes + 1;
