@@ -402,13 +402,15 @@ __device__ __forceinline__ uint64_t inter_decision(const FrameDev& d, int g, int
 // and cleared pass flags.  With summ (not null: after an exhaustive inter
 // search) the blocks of inter frames first take their inter decisions from
 // the chunk summaries and write them to inter_dec; otherwise inter_dec holds
-// them already (learned proposals).
+// them already (learned proposals).  With orig_ready the blocks of inter
+// frames already have their own error in orig (the pruned inter search).
 __global__ void __launch_bounds__(256) k_intra_prep(const FrameDev* __restrict__ fr, const uint8_t* __restrict__ sid, int total,
                              const uint64_t* __restrict__ own, uint64_t* __restrict__ inter_dec,
                              uint8_t* __restrict__ need, uint32_t* __restrict__ cw, float4* __restrict__ ctab,
                              uint32_t* __restrict__ dirty, int* __restrict__ orig, BoundTab* __restrict__ cbt,
                              uint4* __restrict__ pm4, int n_chunks, int S, int thr,
-                             const uint32_t* __restrict__ rw, const uint4* __restrict__ summ, size_t sstride) {
+                             const uint32_t* __restrict__ rw, const uint4* __restrict__ summ, size_t sstride,
+                             int orig_ready) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const FrameDev& d = fr[sid[g]];
@@ -440,6 +442,7 @@ __global__ void __launch_bounds__(256) k_intra_prep(const FrameDev* __restrict__
     pm4[2 * g] = make_uint4(0u, 0u, 0u, 0u);
     pm4[2 * g + 1] = make_uint4(0u, 0u, 0u, 0u);
   }
+  if (orig_ready && !d.is_intra) return;  // the pruned inter search wrote the own error
   BlockPx px;
   load_block(d.rgb, d.width, k % d.bw, k / d.bw, px);
   orig[g] = own_error(px, o);

@@ -595,6 +595,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
   // the exhaustive inter searches leave chunk summaries, from which
   // k_intra_prep takes the inter decisions
   const uint4* inter_summ = nullptr;
+  int orig_ready = 0;
   const unsigned gwk = grid_for(c->n_wtiles * K * 64, wg);
   const unsigned gwk256 = grid_for(c->n_wtiles * K * 64, 256);
   if (!c->any_inter) {
@@ -609,6 +610,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
          })))
       return rc;
     inter_summ = c->d_summ;  // k_intra_prep takes the inter decisions
+    orig_ready = 1;          // and the own errors the inter search wrote
   } else if (c->mode == MPTC_SEARCH_EXHAUSTIVE) {
     if ((rc = launch(c, s, KF_INTER, [&] {
            hipLaunchKernelGGL(k_inter, dim3(gwk), dim3(wg), 0, s, c->d_fr, c->d_tsid, c->n_wtiles, tw, K, S,
@@ -635,7 +637,7 @@ int queue_prefix(mptc_ctx* c, hipStream_t s, bool fit) {
                             c->d_own, c->d_inter, c->d_need, c->d_cw, lrn ? nullptr : c->d_ctab,
                             lrn ? nullptr : c->d_dirty, lrn ? nullptr : c->d_orig, prune ? c->d_cbt : nullptr,
                             prune ? reinterpret_cast<uint4*>(c->d_pm) : nullptr, K, S, thr, c->d_rw, inter_summ,
-                            c->nbs);
+                            c->nbs, orig_ready);
        })))
     return rc;
   if (c->mode == MPTC_SEARCH_LEARNED) {

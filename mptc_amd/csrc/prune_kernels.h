@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
                                                        const uint8_t* __restrict__ need,
                                                        const uint32_t* __restrict__ words,
                                                        const float4* __restrict__ tabs,
-                                                       const BoundTab* __restrict__ btab, const int* __restrict__ orig,
+                                                       const BoundTab* __restrict__ btab, int* __restrict__ orig,
                                                        uint2* __restrict__ E4, size_t nbs, uint32_t* __restrict__ pm,
                                                        uint4* __restrict__ summ, DevStats* __restrict__ st,
                                                        uint64_t* __restrict__ dec, uint32_t* __restrict__ cw_new,
@@ -200,6 +200,7 @@ __global__ void __launch_bounds__(256) k_pruned_search(const FrameDev* __restric
     BlockPx px;
     load_block(d.rgb, d.width, bx, by, px);
     const int e_own = kIntra ? orig[g] : own_error(px, own[g]);
+    if (!kIntra && orig && chunk == 0) orig[g] = e_own;  // inter: k_intra_prep then skips the block's own error
     BoundBlk b;
     make_bound_blk(px, e_own, thr, b);
 #pragma unroll
